@@ -1,0 +1,146 @@
+/*
+ * mclics_hip.h - C ABI of the MI355X (gfx950) checkerboard Metropolis engine.
+ *
+ * Plain C linkage, plain pointers and sizes, no HIP or C++ types.  Every
+ * function returns 0 on success and a negative MC_E* code on failure;
+ * mc_last_error() returns the message of the last failure on the calling
+ * thread.  Nothing here calls exit().
+ *
+ * Host arrays handed in use the reference's layout (include/define.h:6-12 of
+ * the reference): directors AoS float[3*Nt] at (i + Nx*(j + Ny*k))*3 + c,
+ * point types int32[Nt] (0 inactive, 1 bulk, >= 2 surface index + 2) and
+ * surface normals AoS float[3*Nt].  On the device the lattice is kept as three
+ * FP32 SoA planes.
+ */
+#ifndef MCLICS_HIP_H_
+#define MCLICS_HIP_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC_MAX_SURF 8
+
+enum mc_status {
+  MC_OK = 0,
+  MC_EINVAL = -1,  /* bad argument or unsupported configuration */
+  MC_EHIP = -2,    /* HIP runtime error (message names the call) */
+  MC_ENOMEM = -3,  /* device allocation failed */
+  MC_ENODEV = -4,  /* no usable gfx950 device */
+  MC_ESTATE = -5   /* call not valid in the handle's current state */
+};
+
+enum mc_boundary { MC_FREE = 0, MC_PERIODIC = 1 };
+enum mc_potential { MC_LL = 0, MC_GHRL = 1, MC_PEAR = 2 };
+enum mc_anchoring {
+  MC_HOMEOTROPIC = 0,
+  MC_RP = 1,
+  MC_FG = 2,
+  MC_STRONG = 3,
+  MC_HOMEOTROPIC_GHRL = 4,
+  MC_RP_GHRL = 5,
+  MC_FG_GHRL = 6,
+  MC_STRONG_GHRL = 7
+};
+/* Sweep orders.  MC_ORDER_REF8 visits the 8 sublattices of 2x2x2 cells in the
+ * reference CPU order tick -> (tick%2, (tick/2)%2, tick/4)
+ * (src/evolve.cpp:100-102); MC_ORDER_CHECKER updates the two (i+j+k) parity
+ * colours, a different but equally valid Markov chain. */
+enum mc_order { MC_ORDER_CHECKER = 0, MC_ORDER_REF8 = 1 };
+
+/* POD flattening of the reference's Parameters (include/parameters.h:11-83)
+ * restricted to what the sweep reads.  The std::map surface tables become
+ * fixed arrays indexed by surface id. */
+typedef struct mc_params {
+  int32_t nx, ny, nz;
+  int32_t bound[3];          /* mc_boundary per axis */
+  int32_t potential;         /* mc_potential */
+  float A, B1, B2, C;        /* B2 is parsed by the reference but unused */
+  float ghrl_rho, ghrl_lambda, ghrl_mu, ghrl_nu, ghrl_sigma;
+  float elecX, elecY, elecZ, elecA, elecE;
+  float T;
+  int32_t n_surf;
+  int32_t surf_kind[MC_MAX_SURF]; /* mc_anchoring */
+  float surf_W[MC_MAX_SURF];
+  float surf_dir[MC_MAX_SURF][3]; /* (theta_s, phi_s) direction, rp/strong kinds */
+  uint64_t seed;                  /* keys the counter-based generator */
+  uint32_t stream;                /* independent stream id under one seed */
+  int32_t order;                  /* mc_order */
+  float ang_var;                  /* initial proposal width, 0.5 in every run() */
+} mc_params;
+
+typedef struct mc_handle mc_handle;
+
+const char *mc_last_error(void);
+/* Number of visible gfx950 devices (0 when none); never fails. */
+int mc_device_count(void);
+
+/* Replaces the device allocation and upload done at the start of every
+ * reference run(): src/evolve_thermal.cu:45-64 (and step:42-63,
+ * quench:157-178, electric:279-299).  ns may be NULL when no site has
+ * pt >= 2. */
+int mc_create(const mc_params *params, const float *ni_aos, const int32_t *pt, const float *ns_aos,
+              mc_handle **out);
+int mc_destroy(mc_handle *h);
+
+/* Temperature and field strength as read by the next sweep
+ * (src/evolve_thermal.cu:80 H2D of T; src/evolve_electric.cu:82 resend of
+ * Parameters with a new elecE). */
+int mc_set_T(mc_handle *h, float T);
+int mc_set_field(mc_handle *h, float elecE);
+int mc_set_ang_var(mc_handle *h, float ang_var);
+int mc_get_ang_var(mc_handle *h, float *ang_var);
+
+/* n full sweeps (EvolveNGPU::Monte_Carlo_Step_GPU, src/evolve.cu:265-388),
+ * asynchronous on the handle's stream, with the ang_var adaptation done on
+ * the device after each sweep so no host round trip is needed. */
+int mc_sweeps(mc_handle *h, int n);
+/* Same, captured once into a hipGraph per distinct n and replayed. */
+int mc_sweeps_graph(mc_handle *h, int n);
+/* Blocks until queued work is done. */
+int mc_sync(mc_handle *h);
+/* Runs n sweeps bracketed by HIP events on the handle's stream; returns the
+ * elapsed milliseconds and the number of sweep-kernel launches made. */
+int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches);
+
+/* Runs n sweeps eagerly with a HIP event pair around every pass launch on
+ * the handle's stream: the mean pass-kernel duration, the wall time of the
+ * whole region and the number of pass launches. */
+int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, int *passes);
+
+/* Mean site energy and order parameter, FP64 reductions on the device
+ * (energy_calculator_GPU src/evolve.cu:390-436 plus Matrice_constructor /
+ * Eigen_value_evaluation src/parameter_order.cpp:11-60).  E is the sum of
+ * FP32 site energies over active sites divided by their count; q6 (may be
+ * NULL) receives {Qxx,Qxy,Qxz,Qyy,Qyz,Qzz}. */
+int mc_measure(mc_handle *h, double *E, double *S, double *q6);
+/* FP32 site energies of every site (0 at pt==0 sites), float[Nt]. */
+int mc_site_energies(mc_handle *h, float *out);
+
+/* Lattice state transfer, AoS float[3*Nt] (src/evolve_thermal.cu:90,98). */
+int mc_get_directors_aos(mc_handle *h, float *ni_aos);
+int mc_set_directors_aos(mc_handle *h, const float *ni_aos);
+
+/* Accepted count of the last completed sweep and the sweep counter. */
+int mc_get_acceptance(mc_handle *h, uint32_t *accepted, uint32_t *sweeps_done);
+
+/* Parity hook: one pass over a site set with injected proposals (AoS
+ * float[3*Nt]) and uniforms (float[Nt]) at the current T; set 0..7 is a
+ * 2x2x2 sublattice tick, 8/9 a checkerboard colour.  accept/e_old/e_new are
+ * float[Nt]/uint8[Nt] host arrays (any may be NULL) written at the sites of
+ * the set.  Mirrors one callMCGPU launch (src/evolve.cu:93-203) with the CPU
+ * accept rule u < exp(-dE/T) (src/evolve.cpp:160). */
+int mc_pass(mc_handle *h, int set, const float *prop_aos, const float *u, uint8_t *accept, float *e_old,
+            float *e_new);
+
+/* One production sweep that also records, per site, the three uniforms, the
+ * proposal, the accept flag and both energies (host arrays, any NULL), so the
+ * CPU oracle can replay it pass by pass. */
+int mc_record_sweep(mc_handle *h, float *u0, float *u1, float *u2, float *prop_aos, uint8_t *accept,
+                    float *e_old, float *e_new);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,606 @@
+// C ABI of the checkerboard Metropolis engine (declared in include/mclics_hip.h).
+//
+// One handle owns one lattice on the device that was current at mc_create, a
+// private HIP stream (default priority) and the small device state that makes
+// sweeps host-sync free.  Errors are returned as codes with a thread-local
+// message; no function terminates the process.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mclics_hip.h"
+#include "mc_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define MC_HIP(call)                                                                                       \
+  do {                                                                                                     \
+    hipError_t e_ = (call);                                                                                \
+    if (e_ != hipSuccess) {                                                                                \
+      (void)hipGetLastError();                                                                             \
+      return fail(e_ == hipErrorOutOfMemory ? MC_ENOMEM : MC_EHIP,                                         \
+                  std::string(#call) + " failed: " + hipGetErrorString(e_));                              \
+    }                                                                                                      \
+  } while (0)
+
+#define MC_LAUNCHED(what)                                                                                  \
+  do {                                                                                                     \
+    hipError_t e_ = hipGetLastError();                                                                     \
+    if (e_ != hipSuccess) return fail(MC_EHIP, std::string(what) + " launch failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+__global__ void k_set_state_f(MCDevState *st, int field, float v) {
+  if (field == 0) st->T = v;
+  if (field == 1) st->elecE = v;
+  if (field == 2) st->ang_var = v;
+}
+
+__global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var) {
+  st->T = T;
+  st->elecE = elecE;
+  st->ang_var = ang_var;
+  st->sweep = 0u;
+  st->acc_count = 0u;
+  st->blocks_done = 0u;
+  st->last_acc = 0u;
+  st->pad = 0u;
+}
+
+}  // namespace
+
+struct mc_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  MCDevParams P{};
+  int order = MC_ORDER_CHECKER;
+  int cu_count = 256;
+  float *nx = nullptr, *ny = nullptr, *nz = nullptr;
+  uint8_t *pt = nullptr;
+  float *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  MCDevState *st = nullptr;
+  double *partials = nullptr;
+  double *result = nullptr;
+  float *staging = nullptr;  // AoS float[3*Nt]
+  int measure_grid = 0;
+  // Parity and record buffers, allocated on first use.
+  float *pbuf = nullptr;     // u / eo / en / records: 7 float planes
+  uint8_t *abuf = nullptr;   // accept flags
+  std::map<int, hipGraphExec_t> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  MCLaunch launch() const {
+    MCLaunch L;
+    L.P = P;
+    L.stream = stream;
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    L.pt = pt;
+    L.sx = sx;
+    L.sy = sy;
+    L.sz = sz;
+    L.st = st;
+    return L;
+  }
+  int grid_for(int total) const {
+    long long g = ((long long)total + 255) / 256;
+    long long cap = (long long)cu_count * 8;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+  }
+};
+
+namespace {
+
+void free_handle(mc_handle *h) {
+  if (!h) return;
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  void *ptrs[] = {h->nx, h->ny, h->nz, h->pt, h->sx, h->sy, h->sz, h->st, h->partials, h->result, h->staging,
+                  h->pbuf, h->abuf};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int check_handle(mc_handle *h) {
+  if (!h) return fail(MC_EINVAL, "null handle");
+  if (hipSetDevice(h->device) != hipSuccess) return fail(MC_EHIP, "hipSetDevice failed for the handle's device");
+  return MC_OK;
+}
+
+int enqueue_sweep(mc_handle *h, const MCRecord *rec) {
+  const MCLaunch L = h->launch();
+  if (h->order == MC_ORDER_REF8) {
+    for (int tick = 0; tick < 8; ++tick)
+      mc_launch_sweep_pass(L, tick, h->grid_for(mc_set_slots(h->P, tick)), tick == 7, rec);
+  } else {
+    mc_launch_sweep_pass(L, 8, h->grid_for(mc_set_slots(h->P, 8)), 0, rec);
+    mc_launch_sweep_pass(L, 9, h->grid_for(mc_set_slots(h->P, 9)), 1, rec);
+  }
+  MC_LAUNCHED("sweep pass");
+  return MC_OK;
+}
+
+int ensure_parity_buffers(mc_handle *h) {
+  const size_t nt = (size_t)h->P.nt;
+  if (!h->pbuf) MC_HIP(hipMalloc(&h->pbuf, 7 * nt * sizeof(float)));
+  if (!h->abuf) MC_HIP(hipMalloc(&h->abuf, nt));
+  return MC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mc_last_error(void) { return g_err.c_str(); }
+
+int mc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int good = 0;
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) == 0) ++good;
+  }
+  return good;
+}
+
+int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const float *ns_aos, mc_handle **out) {
+  if (!p || !ni_aos || !pt || !out) return fail(MC_EINVAL, "mc_create: null argument");
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(MC_EINVAL, "mc_create: lattice dimensions must be >= 1");
+  const long long ntl = (long long)p->nx * p->ny * p->nz;
+  if (ntl >= (1LL << 31)) return fail(MC_EINVAL, "mc_create: lattice of 2^31 sites or more is not supported");
+  const int dims[3] = {p->nx, p->ny, p->nz};
+  for (int a = 0; a < 3; ++a) {
+    if (p->bound[a] != MC_FREE && p->bound[a] != MC_PERIODIC)
+      return fail(MC_EINVAL, "mc_create: boundary must be MC_FREE or MC_PERIODIC");
+    // An odd periodic extent puts i=0 and i=N-1 (neighbours) on the same
+    // sublattice and colour (reference src/evolve.cpp:105-113), so a pass
+    // would update interacting sites concurrently.
+    if (p->bound[a] == MC_PERIODIC && (dims[a] % 2) != 0)
+      return fail(MC_EINVAL, "mc_create: a periodic axis needs an even number of sites");
+  }
+  if (p->potential < MC_LL || p->potential > MC_PEAR) return fail(MC_EINVAL, "mc_create: unknown potential");
+  if (p->n_surf < 0 || p->n_surf > MC_MAX_SURF) return fail(MC_EINVAL, "mc_create: n_surf out of range");
+  for (int s = 0; s < p->n_surf; ++s)
+    if (p->surf_kind[s] < MC_HOMEOTROPIC || p->surf_kind[s] > MC_STRONG_GHRL)
+      return fail(MC_EINVAL, "mc_create: unknown anchoring kind");
+  if (p->order != MC_ORDER_CHECKER && p->order != MC_ORDER_REF8)
+    return fail(MC_EINVAL, "mc_create: unknown sweep order");
+  if (!(p->T > 0.0f) || !isfinite(p->T)) return fail(MC_EINVAL, "mc_create: temperature must be finite and > 0");
+  const int nt = (int)ntl;
+  bool all_bulk = true, any_surface = false;
+  int valid = 0;
+  for (int s = 0; s < nt; ++s) {
+    if (pt[s] < 0) return fail(MC_EINVAL, "mc_create: negative point type");
+    if (pt[s] > 1) {
+      if (pt[s] - 2 >= p->n_surf) return fail(MC_EINVAL, "mc_create: point type names an undefined surface");
+      any_surface = true;
+    }
+    if (pt[s] > 255) return fail(MC_EINVAL, "mc_create: point type above 255");
+    if (pt[s] != 1) all_bulk = false;
+    if (pt[s]) {
+      ++valid;
+      const float x = ni_aos[3 * (size_t)s], y = ni_aos[3 * (size_t)s + 1], z = ni_aos[3 * (size_t)s + 2];
+      const float n2 = x * x + y * y + z * z;
+      if (!isfinite(n2) || fabsf(n2 - 1.0f) > 1e-3f)
+        return fail(MC_EINVAL, "mc_create: director at site " + std::to_string(s) + " is not a finite unit vector");
+    }
+  }
+  if (valid == 0) return fail(MC_EINVAL, "mc_create: no active site");
+  if (any_surface && !ns_aos) return fail(MC_EINVAL, "mc_create: surface sites need surface normals");
+  if (mc_device_count() < 1) return fail(MC_ENODEV, "mc_create: no gfx950 device visible");
+
+  mc_handle *h = new mc_handle();
+  int rc = MC_OK;
+  auto bail = [&](int code) {
+    free_handle(h);
+    return code;
+  };
+#define MC_HIPC(call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (void)hipGetLastError();                                                              \
+      rc = fail(e_ == hipErrorOutOfMemory ? MC_ENOMEM : MC_EHIP,                            \
+                std::string(#call) + " failed: " + hipGetErrorString(e_));                 \
+      return bail(rc);                                                                      \
+    }                                                                                       \
+  } while (0)
+
+  MC_HIPC(hipGetDevice(&h->device));
+  hipDeviceProp_t prop;
+  MC_HIPC(hipGetDeviceProperties(&prop, h->device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    rc = fail(MC_ENODEV, std::string("mc_create: current device is ") + prop.gcnArchName + ", not gfx950");
+    return bail(rc);
+  }
+  h->cu_count = prop.multiProcessorCount;
+  MC_HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  MC_HIPC(hipEventCreate(&h->ev0));
+  MC_HIPC(hipEventCreate(&h->ev1));
+
+  MCDevParams &P = h->P;
+  P.nx = p->nx;
+  P.ny = p->ny;
+  P.nz = p->nz;
+  for (int a = 0; a < 3; ++a) P.per[a] = p->bound[a] == MC_PERIODIC;
+  P.potential = p->potential;
+  P.A = p->A;
+  P.B1 = p->B1;
+  P.C = p->C;
+  P.rho = p->ghrl_rho;
+  P.lambda = p->ghrl_lambda;
+  P.mu = p->ghrl_mu;
+  P.nu = p->ghrl_nu;
+  P.sigma = p->ghrl_sigma;
+  P.eX = p->elecX;
+  P.eY = p->elecY;
+  P.eZ = p->elecZ;
+  P.elecA = p->elecA;
+  P.n_surf = p->n_surf;
+  for (int s = 0; s < MC_MAX_SURF; ++s) {
+    P.surf_kind[s] = s < p->n_surf ? p->surf_kind[s] : 0;
+    P.surf_W[s] = s < p->n_surf ? p->surf_W[s] : 0.0f;
+    for (int c = 0; c < 3; ++c) P.surf_dir[s][c] = s < p->n_surf ? p->surf_dir[s][c] : 0.0f;
+  }
+  P.seed_lo = (uint32_t)(p->seed & 0xFFFFFFFFu);
+  P.seed_hi = (uint32_t)(p->seed >> 32);
+  P.stream = p->stream;
+  P.nt = nt;
+  P.valid = valid;
+  h->order = p->order;
+
+  const size_t fbytes = (size_t)nt * sizeof(float);
+  MC_HIPC(hipMalloc(&h->nx, fbytes));
+  MC_HIPC(hipMalloc(&h->ny, fbytes));
+  MC_HIPC(hipMalloc(&h->nz, fbytes));
+  MC_HIPC(hipMalloc(&h->staging, 3 * fbytes));
+  MC_HIPC(hipMalloc(&h->st, sizeof(MCDevState)));
+  h->measure_grid = h->grid_for(nt);
+  MC_HIPC(hipMalloc(&h->partials, (size_t)h->measure_grid * 8 * sizeof(double)));
+  MC_HIPC(hipMalloc(&h->result, 16 * sizeof(double)));
+  const int cgrid = h->grid_for(nt);
+  MC_HIPC(hipMemcpyAsync(h->staging, ni_aos, 3 * fbytes, hipMemcpyHostToDevice, h->stream));
+  mc_launch_aos_to_soa(h->stream, h->staging, h->nx, h->ny, h->nz, nt, cgrid);
+  if (!all_bulk) {
+    std::vector<uint8_t> pt8(nt);
+    for (int s = 0; s < nt; ++s) pt8[s] = (uint8_t)pt[s];
+    MC_HIPC(hipMalloc(&h->pt, (size_t)nt));
+    MC_HIPC(hipMemcpyAsync(h->pt, pt8.data(), (size_t)nt, hipMemcpyHostToDevice, h->stream));
+    MC_HIPC(hipStreamSynchronize(h->stream));
+  }
+  if (any_surface) {
+    MC_HIPC(hipMalloc(&h->sx, fbytes));
+    MC_HIPC(hipMalloc(&h->sy, fbytes));
+    MC_HIPC(hipMalloc(&h->sz, fbytes));
+    float *tmp = nullptr;
+    MC_HIPC(hipMalloc(&tmp, 3 * fbytes));
+    hipError_t e1 = hipMemcpyAsync(tmp, ns_aos, 3 * fbytes, hipMemcpyHostToDevice, h->stream);
+    if (e1 == hipSuccess) mc_launch_aos_to_soa(h->stream, tmp, h->sx, h->sy, h->sz, nt, cgrid);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    MC_HIPC(e1);
+    MC_HIPC(e2);
+  }
+  k_init_state<<<1, 1, 0, h->stream>>>(h->st, p->T, p->elecE, p->ang_var);
+  MC_HIPC(hipGetLastError());
+  MC_HIPC(hipStreamSynchronize(h->stream));
+#undef MC_HIPC
+  *out = h;
+  return MC_OK;
+}
+
+int mc_destroy(mc_handle *h) {
+  if (!h) return MC_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_handle(h);
+  return MC_OK;
+}
+
+static int set_state(mc_handle *h, int field, float v) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  k_set_state_f<<<1, 1, 0, h->stream>>>(h->st, field, v);
+  MC_LAUNCHED("set state");
+  return MC_OK;
+}
+
+int mc_set_T(mc_handle *h, float T) {
+  if (!(T > 0.0f) || !isfinite(T)) return fail(MC_EINVAL, "mc_set_T: temperature must be finite and > 0");
+  return set_state(h, 0, T);
+}
+int mc_set_field(mc_handle *h, float elecE) {
+  if (!isfinite(elecE)) return fail(MC_EINVAL, "mc_set_field: field must be finite");
+  return set_state(h, 1, elecE);
+}
+int mc_set_ang_var(mc_handle *h, float a) {
+  if (!isfinite(a) || a < 0.0f) return fail(MC_EINVAL, "mc_set_ang_var: ang_var must be finite and >= 0");
+  return set_state(h, 2, a);
+}
+
+int mc_get_ang_var(mc_handle *h, float *a) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!a) return fail(MC_EINVAL, "mc_get_ang_var: null output");
+  MCDevState s;
+  MC_HIP(hipMemcpyAsync(&s, h->st, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  *a = s.ang_var;
+  return MC_OK;
+}
+
+int mc_get_acceptance(mc_handle *h, uint32_t *accepted, uint32_t *sweeps_done) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  MCDevState s;
+  MC_HIP(hipMemcpyAsync(&s, h->st, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  if (accepted) *accepted = s.last_acc;
+  if (sweeps_done) *sweeps_done = s.sweep;
+  return MC_OK;
+}
+
+int mc_sweeps(mc_handle *h, int n) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (n < 0) return fail(MC_EINVAL, "mc_sweeps: negative sweep count");
+  for (int w = 0; w < n; ++w) {
+    rc = enqueue_sweep(h, nullptr);
+    if (rc) return rc;
+  }
+  return MC_OK;
+}
+
+int mc_sweeps_graph(mc_handle *h, int n) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (n < 0) return fail(MC_EINVAL, "mc_sweeps_graph: negative sweep count");
+  if (n == 0) return MC_OK;
+  auto it = h->graphs.find(n);
+  if (it == h->graphs.end()) {
+    hipGraph_t g = nullptr;
+    MC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int w = 0; w < n; ++w) {
+      rc = enqueue_sweep(h, nullptr);
+      if (rc) {
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+    }
+    MC_HIP(hipStreamEndCapture(h->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MC_HIP(e);
+    it = h->graphs.emplace(n, ge).first;
+  }
+  MC_HIP(hipGraphLaunch(it->second, h->stream));
+  return MC_OK;
+}
+
+int mc_sync(mc_handle *h) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (n < 1 || !ms) return fail(MC_EINVAL, "mc_time_sweeps: need n >= 1 and an output");
+  if (use_graph) {
+    // Instantiate before timing so capture cost stays outside the region.
+    if (h->graphs.find(n) == h->graphs.end()) {
+      MC_HIP(hipStreamSynchronize(h->stream));
+      hipGraph_t g = nullptr;
+      MC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int w = 0; w < n; ++w) {
+        rc = enqueue_sweep(h, nullptr);
+        if (rc) {
+          (void)hipStreamEndCapture(h->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+      }
+      MC_HIP(hipStreamEndCapture(h->stream, &g));
+      hipGraphExec_t ge = nullptr;
+      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      MC_HIP(e);
+      h->graphs.emplace(n, ge);
+    }
+  }
+  MC_HIP(hipEventRecord(h->ev0, h->stream));
+  rc = use_graph ? mc_sweeps_graph(h, n) : mc_sweeps(h, n);
+  if (rc) return rc;
+  MC_HIP(hipEventRecord(h->ev1, h->stream));
+  MC_HIP(hipEventSynchronize(h->ev1));
+  MC_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1));
+  if (launches) *launches = n * (h->order == MC_ORDER_REF8 ? 8 : 2);
+  return MC_OK;
+}
+
+int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, int *passes) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (n < 1 || !avg_pass_ms) return fail(MC_EINVAL, "mc_time_passes: need n >= 1 and an output");
+  const int per = h->order == MC_ORDER_REF8 ? 8 : 2;
+  const int np = n * per;
+  std::vector<hipEvent_t> ev(2 * (size_t)np, nullptr);
+  for (auto &e : ev) MC_HIP(hipEventCreate(&e));
+  const MCLaunch L = h->launch();
+  MC_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int w = 0, q = 0; w < n; ++w) {
+    for (int t = 0; t < per; ++t, ++q) {
+      const int set = per == 8 ? t : 8 + t;
+      MC_HIP(hipEventRecord(ev[2 * q], h->stream));
+      mc_launch_sweep_pass(L, set, h->grid_for(mc_set_slots(h->P, set)), t == per - 1, nullptr);
+      MC_HIP(hipEventRecord(ev[2 * q + 1], h->stream));
+    }
+  }
+  MC_LAUNCHED("timed sweep pass");
+  MC_HIP(hipEventRecord(h->ev1, h->stream));
+  MC_HIP(hipEventSynchronize(h->ev1));
+  double sum = 0.0;
+  for (int q = 0; q < np; ++q) {
+    float ms = 0.0f;
+    MC_HIP(hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+    sum += ms;
+  }
+  float tot = 0.0f;
+  MC_HIP(hipEventElapsedTime(&tot, h->ev0, h->ev1));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *avg_pass_ms = sum / np;
+  if (total_ms) *total_ms = tot;
+  if (passes) *passes = np;
+  return MC_OK;
+}
+
+int mc_measure(mc_handle *h, double *E, double *S, double *q6) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result);
+  MC_LAUNCHED("measure");
+  double r[9];
+  MC_HIP(hipMemcpyAsync(r, h->result, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  if (E) *E = r[0];
+  if (S) *S = r[1];
+  if (q6)
+    for (int c = 0; c < 6; ++c) q6[c] = r[2 + c];
+  return MC_OK;
+}
+
+int mc_site_energies(mc_handle *h, float *out) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!out) return fail(MC_EINVAL, "mc_site_energies: null output");
+  rc = ensure_parity_buffers(h);
+  if (rc) return rc;
+  mc_launch_site_energy(h->launch(), h->grid_for(h->P.nt), h->pbuf);
+  MC_LAUNCHED("site energy");
+  MC_HIP(hipMemcpyAsync(out, h->pbuf, (size_t)h->P.nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+int mc_get_directors_aos(mc_handle *h, float *ni_aos) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!ni_aos) return fail(MC_EINVAL, "mc_get_directors_aos: null output");
+  const int nt = h->P.nt;
+  mc_launch_soa_to_aos(h->stream, h->nx, h->ny, h->nz, h->staging, nt, h->grid_for(nt));
+  MC_LAUNCHED("soa_to_aos");
+  MC_HIP(hipMemcpyAsync(ni_aos, h->staging, 3 * (size_t)nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+int mc_set_directors_aos(mc_handle *h, const float *ni_aos) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!ni_aos) return fail(MC_EINVAL, "mc_set_directors_aos: null input");
+  const int nt = h->P.nt;
+  MC_HIP(hipMemcpyAsync(h->staging, ni_aos, 3 * (size_t)nt * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  mc_launch_aos_to_soa(h->stream, h->staging, h->nx, h->ny, h->nz, nt, h->grid_for(nt));
+  MC_LAUNCHED("aos_to_soa");
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+int mc_pass(mc_handle *h, int set, const float *prop_aos, const float *u, uint8_t *accept, float *e_old,
+            float *e_new) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (set < 0 || set > 9) return fail(MC_EINVAL, "mc_pass: set must be 0..7 (sublattice) or 8/9 (colour)");
+  if (!prop_aos || !u) return fail(MC_EINVAL, "mc_pass: proposals and uniforms are required");
+  rc = ensure_parity_buffers(h);
+  if (rc) return rc;
+  const size_t nt = (size_t)h->P.nt;
+  float *d_u = h->pbuf, *d_eo = h->pbuf + nt, *d_en = h->pbuf + 2 * nt;
+  MC_HIP(hipMemcpyAsync(h->staging, prop_aos, 3 * nt * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  MC_HIP(hipMemcpyAsync(d_u, u, nt * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  MC_HIP(hipMemsetAsync(d_eo, 0, 2 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->abuf, 0, nt, h->stream));
+  mc_launch_pass_inject(h->launch(), set, h->grid_for(mc_set_slots(h->P, set)), h->staging, d_u, h->abuf, d_eo,
+                        d_en);
+  MC_LAUNCHED("pass_inject");
+  if (accept) MC_HIP(hipMemcpyAsync(accept, h->abuf, nt, hipMemcpyDeviceToHost, h->stream));
+  if (e_old) MC_HIP(hipMemcpyAsync(e_old, d_eo, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (e_new) MC_HIP(hipMemcpyAsync(e_new, d_en, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+int mc_record_sweep(mc_handle *h, float *u0, float *u1, float *u2, float *prop_aos, uint8_t *accept, float *e_old,
+                    float *e_new) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  rc = ensure_parity_buffers(h);
+  if (rc) return rc;
+  const size_t nt = (size_t)h->P.nt;
+  // Planes: u0 u1 u2 eo en in pbuf, proposals in staging (SoA thirds).
+  MCRecord rec;
+  rec.u0 = h->pbuf;
+  rec.u1 = h->pbuf + nt;
+  rec.u2 = h->pbuf + 2 * nt;
+  rec.eo = h->pbuf + 3 * nt;
+  rec.en = h->pbuf + 4 * nt;
+  rec.px = h->staging;
+  rec.py = h->staging + nt;
+  rec.pz = h->staging + 2 * nt;
+  rec.acc = h->abuf;
+  MC_HIP(hipMemsetAsync(h->pbuf, 0, 5 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->staging, 0, 3 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->abuf, 0, nt, h->stream));
+  rc = enqueue_sweep(h, &rec);
+  if (rc) return rc;
+  if (u0) MC_HIP(hipMemcpyAsync(u0, rec.u0, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (u1) MC_HIP(hipMemcpyAsync(u1, rec.u1, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (u2) MC_HIP(hipMemcpyAsync(u2, rec.u2, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (e_old) MC_HIP(hipMemcpyAsync(e_old, rec.eo, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (e_new) MC_HIP(hipMemcpyAsync(e_new, rec.en, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (accept) MC_HIP(hipMemcpyAsync(accept, rec.acc, nt, hipMemcpyDeviceToHost, h->stream));
+  if (prop_aos) {
+    // Interleave the SoA proposal planes into the caller's AoS array via the
+    // pbuf tail (2 spare planes are not enough for 3, so go through host).
+    std::vector<float> soa(3 * nt);
+    MC_HIP(hipMemcpyAsync(soa.data(), h->staging, 3 * nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    MC_HIP(hipStreamSynchronize(h->stream));
+    for (size_t s = 0; s < nt; ++s) {
+      prop_aos[3 * s] = soa[s];
+      prop_aos[3 * s + 1] = soa[nt + s];
+      prop_aos[3 * s + 2] = soa[2 * nt + s];
+    }
+  }
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// Device-side physics of the checkerboard Metropolis sweep (gfx950).
+//
+// Every energy expression reproduces the reference CPU path operation by
+// operation: FP32 dot products evaluated ((a0*b0 + a1*b1) + a2*b2) with no
+// FMA contraction (this translation unit is compiled with -ffp-contract=off),
+// the double literals of the LL/GHRL/pear bonds kept as FP64 arithmetic, and
+// the six-bond sum accumulated in FP64 and rounded once to FP32
+// (src/geometry.cpp:19-72).  CDNA4 runs FP64 FMA/mul at the same issue rate as
+// unpacked FP32, so keeping the reference's mixed precision costs conversion
+// instructions only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MCD_MAX_SURF 8
+
+enum { MCD_LL = 0, MCD_GHRL = 1, MCD_PEAR = 2 };
+enum {
+  MCD_HOMEOTROPIC = 0,
+  MCD_RP = 1,
+  MCD_FG = 2,
+  MCD_STRONG = 3,
+  MCD_HOMEOTROPIC_GHRL = 4,
+  MCD_RP_GHRL = 5,
+  MCD_FG_GHRL = 6,
+  MCD_STRONG_GHRL = 7
+};
+
+// Immutable per-handle parameters, passed by value as a kernel argument.
+struct MCDevParams {
+  int nx, ny, nz;
+  int per[3];  // 1 periodic, 0 free
+  int potential;
+  float A, B1, C;
+  float rho, lambda, mu, nu, sigma;
+  float eX, eY, eZ, elecA;
+  int n_surf;
+  int surf_kind[MCD_MAX_SURF];
+  float surf_W[MCD_MAX_SURF];
+  float surf_dir[MCD_MAX_SURF][3];
+  uint32_t seed_lo, seed_hi, stream;
+  int nt;
+  int valid;
+};
+
+// Mutable sweep state in device memory, so captured graphs replay with the
+// current temperature, field and proposal width.
+struct MCDevState {
+  float T;
+  float elecE;
+  float ang_var;
+  uint32_t sweep;        // index of the next sweep (RNG counter word 1)
+  uint32_t acc_count;    // accepted moves of the sweep in progress
+  uint32_t blocks_done;  // last-block detection of the final pass
+  uint32_t last_acc;     // accepted moves of the last completed sweep
+  uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): two 32x32->64 products per round,
+// Weyl key increments between rounds.  (uint64_t)a*b lowers to
+// v_mad_u64_u32, one instruction for both product halves.
+struct MCU4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ MCU4 mcd_philox(MCU4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    MCU4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+  }
+  return c;
+}
+
+// 24-bit uniform on [0,1), exact in FP32.
+__device__ __forceinline__ float mcd_u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------------------
+// Energy terms.
+
+__device__ __forceinline__ float mcd_dot(float a0, float a1, float a2, float b0, float b1, float b2) {
+  float t = a0 * b0;
+  t = t + a1 * b1;
+  t = t + a2 * b2;
+  return t;
+}
+
+// src/potential.cpp:210-213: -A*1.5*nij*nij evaluated in double.
+__device__ __forceinline__ float mcd_bond_ll(double cA, float nij) {
+  double v = cA * (double)nij;
+  v = v * (double)nij;
+  return (float)v;
+}
+
+// src/potential.cpp:215-231 (CPU form: E1 built in double, em*(...-1.0/9) in
+// double, the sum in double).
+__device__ __forceinline__ float mcd_bond_ghrl(const MCDevParams &P, float n0, float n1, float n2, float m0,
+                                               float m1, float m2, float r0, float r1, float r2) {
+  float ai = mcd_dot(n0, n1, n2, r0, r1, r2);
+  float aj = mcd_dot(m0, m1, m2, r0, r1, r2);
+  float nij = mcd_dot(n0, n1, n2, m0, m1, m2);
+  float pij = 1.5f * nij * nij - 0.5f;
+  float c0 = n2 * m1 - n1 * m2;
+  float c1 = n0 * m2 - n2 * m0;
+  float c2 = n1 * m0 - n0 * m1;
+  float cross = c0 * r0;
+  cross = cross + c1 * r1;
+  cross = cross + c2 * r2;
+  double e1d = 1.5 * (double)ai * (double)ai;
+  e1d = e1d + 1.5 * (double)aj * (double)aj;
+  e1d = e1d - 1.0;
+  float E1 = (float)e1d;
+  float t1 = E1 * (P.rho * pij + P.lambda);
+  float aan = ai * aj * nij;
+  double t2 = (double)P.mu * ((double)aan - 1.0 / 9);
+  double s = (double)t1 + t2;
+  s = s + (double)(P.nu * pij);
+  float sg = nij > 0 ? 1.0f : -1.0f;
+  s = s + (double)(P.sigma * sg * cross);
+  return (float)s;
+}
+
+// src/potential.cpp:203-208.
+__device__ __forceinline__ float mcd_bond_pear(const MCDevParams &P, float n0, float n1, float n2, float m0,
+                                               float m1, float m2, float r0, float r1, float r2) {
+  float nij = mcd_dot(n0, n1, n2, m0, m1, m2);
+  float a = 1.0f + 2.0f * nij;
+  a = a + nij * nij;
+  float d = (m0 - n0) * r0;
+  d = d + (m1 - n1) * r1;
+  d = d + (m2 - n2) * r2;
+  float ps = a * d;
+  float f = -P.A * nij * nij;
+  f = f - P.B1 * nij;
+  double v = (double)f - ((double)P.C * 0.25) * (double)ps;
+  return (float)v;
+}
+
+// src/potential.cpp:232-236.
+__device__ __forceinline__ float mcd_field(const MCDevParams &P, float elecE, float n0, float n1, float n2) {
+  float d = n0 * P.eX;
+  d = d + n1 * P.eY;
+  d = d + n2 * P.eZ;
+  float nde = elecE * d;
+  return -P.elecA * (nde * nde);
+}
+
+__device__ __forceinline__ float mcd_ghrl_surface_form(const MCDevParams &P, float W, float n0, float n1, float n2,
+                                                       float m0, float m1, float m2, float s0, float s1, float s2,
+                                                       float ai, float aj) {
+  float nij = mcd_dot(n0, n1, n2, m0, m1, m2);
+  float pij = 1.5f * nij * nij - 0.5f;
+  float c0 = n2 * m1 - n1 * m2;
+  float c1 = n0 * m2 - n2 * m0;
+  float c2 = n1 * m0 - n0 * m1;
+  float cross = c0 * s0;
+  cross = cross + c1 * s1;
+  cross = cross + c2 * s2;
+  float E1 = 1.5f * ai * ai;
+  E1 = E1 + 1.5f * aj * aj;
+  E1 = E1 - 1.0f;
+  float t = E1 * (P.rho * pij + P.lambda);
+  t = t + P.mu * (ai * aj * nij);
+  t = t - 0.0f;  // the reference's integer (1/9)
+  t = t + P.nu * pij;
+  float sg = nij > 0 ? 1.0f : -1.0f;
+  t = t + P.sigma * sg * cross;
+  return W * t;
+}
+
+// Surface terms of the CPU anchoring classes (src/anchoring_homeotropic.cpp,
+// src/anchoring_rp.cpp, src/anchoring_fg.cpp, src/anchoring_strong.cpp).
+__device__ __noinline__ float mcd_surface(const MCDevParams &P, int surf, float n0, float n1, float n2, float s0,
+                                          float s1, float s2) {
+  const float W = P.surf_W[surf];
+  const float d0 = P.surf_dir[surf][0], d1 = P.surf_dir[surf][1], d2 = P.surf_dir[surf][2];
+  switch (P.surf_kind[surf]) {
+    case MCD_HOMEOTROPIC: {
+      float nij = mcd_dot(n0, n1, n2, s0, s1, s2);
+      return -W * nij * nij;
+    }
+    case MCD_RP:
+    case MCD_STRONG:
+    case MCD_STRONG_GHRL: {
+      float nij = mcd_dot(n0, n1, n2, d0, d1, d2);
+      return -W * nij * nij;
+    }
+    case MCD_FG: {
+      float nij = mcd_dot(n0, n1, n2, s0, s1, s2);
+      return +W * nij * nij;
+    }
+    case MCD_HOMEOTROPIC_GHRL: {
+      float ai = mcd_dot(n0, n1, n2, s0, s1, s2);
+      return mcd_ghrl_surface_form(P, W, n0, n1, n2, s0, s1, s2, s0, s1, s2, ai, 1.0f);
+    }
+    case MCD_RP_GHRL: {
+      float ai = mcd_dot(n0, n1, n2, s0, s1, s2);
+      float aj = mcd_dot(d0, d1, d2, s0, s1, s2);
+      return mcd_ghrl_surface_form(P, W, n0, n1, n2, d0, d1, d2, s0, s1, s2, ai, aj);
+    }
+    case MCD_FG_GHRL: {
+      float mod = sqrtf(fabsf(mcd_dot(n0, n1, n2, s0, s1, s2)));
+      float m0 = n0 - s0 * mod, m1 = n1 - s1 * mod, m2 = n2 - s2 * mod;
+      mod = sqrtf(fabsf(mcd_dot(m0, m1, m2, m0, m1, m2)));
+      float dv = mod > 0 ? mod : 1.0f;
+      m0 = m0 / dv;
+      m1 = m1 / dv;
+      m2 = m2 / dv;
+      float ai = mcd_dot(n0, n1, n2, s0, s1, s2);
+      float aj = mcd_dot(m0, m1, m2, s0, s1, s2);
+      return mcd_ghrl_surface_form(P, W, n0, n1, n2, m0, m1, m2, s0, s1, s2, ai, aj);
+    }
+    default:
+      return 0.0f;
+  }
+}
+
+// Bond energy for neighbour b (0:+x 1:-x 2:+y 3:-y 4:+z 5:-z).
+template <int POT>
+__device__ __forceinline__ float mcd_bond(const MCDevParams &P, double cA, int b, float n0, float n1, float n2,
+                                          float m0, float m1, float m2) {
+  if (POT == MCD_LL) return mcd_bond_ll(cA, mcd_dot(n0, n1, n2, m0, m1, m2));
+  const float r0 = b == 0 ? 1.0f : (b == 1 ? -1.0f : 0.0f);
+  const float r1 = b == 2 ? 1.0f : (b == 3 ? -1.0f : 0.0f);
+  const float r2 = b == 4 ? 1.0f : (b == 5 ? -1.0f : 0.0f);
+  if (POT == MCD_GHRL) return mcd_bond_ghrl(P, n0, n1, n2, m0, m1, m2, r0, r1, r2);
+  return mcd_bond_pear(P, n0, n1, n2, m0, m1, m2, r0, r1, r2);
+}
+
+// Accept rule of the reference CPU sweep, u < expf(-dE/T) (src/evolve.cpp:160).
+// A hardware exp2 decides every case whose margin exceeds its error bound
+// (|x| <= 2^11 keeps the relative error of exp2(x*log2e) under 2^-13); the
+// rare near-ties, u == 0 and infinities are settled with an FP64 exp rounded
+// to FP32, i.e. the correctly rounded expf the CPU returns.
+__device__ __forceinline__ bool mcd_accept(float E_new, float E_old, float T, float u) {
+  float x = -(E_new - E_old) / T;
+  float pf = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  if (u != 0.0f && !(fabsf(u - pf) <= pf * 1.220703125e-4f)) return u < pf;
+  return u < (float)exp((double)x);
+}

@@ -1,0 +1,326 @@
+"""ctypes binding of the HIP engine's C ABI (include/mclics_hip.h).
+
+The engine is the shared library built from ``csrc/`` into
+``lib/libmclics_hip.so``.  Loading fails loudly when it has not been built:
+there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmclics_hip.so")
+
+MC_MAX_SURF = 8
+FREE, PERIODIC = 0, 1
+LL, GHRL, PEAR = 0, 1, 2
+ORDER_CHECKER, ORDER_REF8 = 0, 1
+ANCHORING = {
+    "homeotropic": 0,
+    "rp": 1,
+    "fg": 2,
+    "strong": 3,
+    "homeotropic_ghrl": 4,
+    "rp_ghrl": 5,
+    "fg_ghrl": 6,
+    "strong_ghrl": 7,
+}
+
+# Every symbol the public header declares.
+EXPORTS = (
+    "mc_last_error",
+    "mc_device_count",
+    "mc_create",
+    "mc_destroy",
+    "mc_set_T",
+    "mc_set_field",
+    "mc_set_ang_var",
+    "mc_get_ang_var",
+    "mc_sweeps",
+    "mc_sweeps_graph",
+    "mc_sync",
+    "mc_time_sweeps",
+    "mc_time_passes",
+    "mc_measure",
+    "mc_site_energies",
+    "mc_get_directors_aos",
+    "mc_set_directors_aos",
+    "mc_get_acceptance",
+    "mc_pass",
+    "mc_record_sweep",
+)
+
+
+class McParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("bound", C.c_int32 * 3),
+        ("potential", C.c_int32),
+        ("A", C.c_float),
+        ("B1", C.c_float),
+        ("B2", C.c_float),
+        ("C", C.c_float),
+        ("ghrl_rho", C.c_float),
+        ("ghrl_lambda", C.c_float),
+        ("ghrl_mu", C.c_float),
+        ("ghrl_nu", C.c_float),
+        ("ghrl_sigma", C.c_float),
+        ("elecX", C.c_float),
+        ("elecY", C.c_float),
+        ("elecZ", C.c_float),
+        ("elecA", C.c_float),
+        ("elecE", C.c_float),
+        ("T", C.c_float),
+        ("n_surf", C.c_int32),
+        ("surf_kind", C.c_int32 * MC_MAX_SURF),
+        ("surf_W", C.c_float * MC_MAX_SURF),
+        ("surf_dir", (C.c_float * 3) * MC_MAX_SURF),
+        ("seed", C.c_uint64),
+        ("stream", C.c_uint32),
+        ("order", C.c_int32),
+        ("ang_var", C.c_float),
+    ]
+
+
+class EngineError(RuntimeError):
+    """A non-zero status from the C ABI, with mc_last_error()'s message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the engine library; raises FileNotFoundError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"HIP engine not built: {path} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    f32p = C.POINTER(C.c_float)
+    u8p = C.POINTER(C.c_uint8)
+    vp = C.c_void_p
+    lib.mc_last_error.restype = C.c_char_p
+    lib.mc_device_count.restype = C.c_int
+    lib.mc_create.argtypes = [C.POINTER(McParams), f32p, C.POINTER(C.c_int32), f32p, C.POINTER(vp)]
+    lib.mc_destroy.argtypes = [vp]
+    lib.mc_set_T.argtypes = [vp, C.c_float]
+    lib.mc_set_field.argtypes = [vp, C.c_float]
+    lib.mc_set_ang_var.argtypes = [vp, C.c_float]
+    lib.mc_get_ang_var.argtypes = [vp, f32p]
+    lib.mc_sweeps.argtypes = [vp, C.c_int]
+    lib.mc_sweeps_graph.argtypes = [vp, C.c_int]
+    lib.mc_sync.argtypes = [vp]
+    lib.mc_time_sweeps.argtypes = [vp, C.c_int, C.c_int, f32p, C.POINTER(C.c_int)]
+    lib.mc_time_passes.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mc_site_energies.argtypes = [vp, f32p]
+    lib.mc_get_directors_aos.argtypes = [vp, f32p]
+    lib.mc_set_directors_aos.argtypes = [vp, f32p]
+    lib.mc_get_acceptance.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.mc_pass.argtypes = [vp, C.c_int, f32p, f32p, u8p, f32p, f32p]
+    lib.mc_record_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p, f32p]
+    for name in EXPORTS:
+        if name not in ("mc_last_error", "mc_device_count"):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise EngineError(rc, load_library().mc_last_error().decode())
+
+
+def _f32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _opt_f32(a):
+    return None if a is None else _f32(a)
+
+
+def _u8(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+@dataclass
+class Surface:
+    kind: str
+    W: float
+    direction: Sequence[float] = (0.0, 0.0, 1.0)
+
+
+@dataclass
+class Config:
+    """Parameters of one lattice; names follow the reference's Parameters."""
+
+    nx: int
+    ny: int
+    nz: int
+    bound: Sequence[int] = (PERIODIC, PERIODIC, PERIODIC)
+    potential: int = LL
+    A: float = 1.0
+    B1: float = 0.04
+    B2: float = 0.4
+    C: float = 0.3
+    ghrl: Sequence[float] = (0.0, 0.0, 0.0, 1.0, 0.0)  # rho, lambda, mu, nu, sigma
+    elec: Sequence[float] = (0.0, 0.0, 0.0)
+    elecA: float = 0.0
+    elecE: float = 1.0
+    T: float = 1.0
+    surfaces: Sequence[Surface] = field(default_factory=list)
+    seed: int = 1
+    stream: int = 0
+    order: int = ORDER_CHECKER
+    ang_var: float = 0.5
+
+    def to_struct(self) -> McParams:
+        p = McParams()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        for a in range(3):
+            p.bound[a] = int(self.bound[a])
+        p.potential = self.potential
+        p.A, p.B1, p.B2, p.C = self.A, self.B1, self.B2, self.C
+        (p.ghrl_rho, p.ghrl_lambda, p.ghrl_mu, p.ghrl_nu, p.ghrl_sigma) = self.ghrl
+        p.elecX, p.elecY, p.elecZ = self.elec
+        p.elecA, p.elecE, p.T = self.elecA, self.elecE, self.T
+        p.n_surf = len(self.surfaces)
+        for s, sf in enumerate(self.surfaces):
+            p.surf_kind[s] = ANCHORING[sf.kind]
+            p.surf_W[s] = sf.W
+            for c in range(3):
+                p.surf_dir[s][c] = sf.direction[c]
+        p.seed = self.seed
+        p.stream = self.stream
+        p.order = self.order
+        p.ang_var = self.ang_var
+        return p
+
+
+class Lattice:
+    """One device-resident lattice (an mc_handle)."""
+
+    def __init__(self, cfg: Config, ni_aos: np.ndarray, pt: Optional[np.ndarray] = None,
+                 ns_aos: Optional[np.ndarray] = None):
+        lib = load_library()
+        nt = cfg.nx * cfg.ny * cfg.nz
+        self.cfg = cfg
+        self.nt = nt
+        ni = np.ascontiguousarray(ni_aos, dtype=np.float32).reshape(3 * nt)
+        ptv = np.ones(nt, np.int32) if pt is None else np.ascontiguousarray(pt, dtype=np.int32).reshape(nt)
+        ns = None if ns_aos is None else np.ascontiguousarray(ns_aos, dtype=np.float32).reshape(3 * nt)
+        self._params = cfg.to_struct()
+        h = C.c_void_p()
+        _check(lib.mc_create(C.byref(self._params), _f32(ni), ptv.ctypes.data_as(C.POINTER(C.c_int32)),
+                             _opt_f32(ns), C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.mc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- state -----------------------------------------------------------
+    def set_T(self, T: float) -> None:
+        _check(self._lib.mc_set_T(self._h, T))
+
+    def set_field(self, E: float) -> None:
+        _check(self._lib.mc_set_field(self._h, E))
+
+    def set_ang_var(self, a: float) -> None:
+        _check(self._lib.mc_set_ang_var(self._h, a))
+
+    def ang_var(self) -> float:
+        v = C.c_float()
+        _check(self._lib.mc_get_ang_var(self._h, C.byref(v)))
+        return v.value
+
+    def acceptance(self):
+        a, s = C.c_uint32(), C.c_uint32()
+        _check(self._lib.mc_get_acceptance(self._h, C.byref(a), C.byref(s)))
+        return a.value, s.value
+
+    def directors(self) -> np.ndarray:
+        out = np.empty(3 * self.nt, np.float32)
+        _check(self._lib.mc_get_directors_aos(self._h, _f32(out)))
+        return out
+
+    def set_directors(self, ni_aos: np.ndarray) -> None:
+        ni = np.ascontiguousarray(ni_aos, dtype=np.float32).reshape(3 * self.nt)
+        _check(self._lib.mc_set_directors_aos(self._h, _f32(ni)))
+
+    # --- work --------------------------------------------------------------
+    def sweeps(self, n: int, graph: bool = False) -> None:
+        _check((self._lib.mc_sweeps_graph if graph else self._lib.mc_sweeps)(self._h, n))
+
+    def sync(self) -> None:
+        _check(self._lib.mc_sync(self._h))
+
+    def time_sweeps(self, n: int, graph: bool = True):
+        ms, launches = C.c_float(), C.c_int()
+        _check(self._lib.mc_time_sweeps(self._h, n, 1 if graph else 0, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def time_passes(self, n: int):
+        avg, tot, np_ = C.c_double(), C.c_double(), C.c_int()
+        _check(self._lib.mc_time_passes(self._h, n, C.byref(avg), C.byref(tot), C.byref(np_)))
+        return avg.value, tot.value, np_.value
+
+    def measure(self):
+        E, S = C.c_double(), C.c_double()
+        q = (C.c_double * 6)()
+        _check(self._lib.mc_measure(self._h, C.byref(E), C.byref(S), q))
+        return E.value, S.value, np.array(q[:], np.float64)
+
+    def site_energies(self) -> np.ndarray:
+        out = np.empty(self.nt, np.float32)
+        _check(self._lib.mc_site_energies(self._h, _f32(out)))
+        return out
+
+    def pass_inject(self, set_id: int, prop_aos: np.ndarray, u: np.ndarray):
+        prop = np.ascontiguousarray(prop_aos, dtype=np.float32).reshape(3 * self.nt)
+        uu = np.ascontiguousarray(u, dtype=np.float32).reshape(self.nt)
+        acc = np.zeros(self.nt, np.uint8)
+        eo = np.zeros(self.nt, np.float32)
+        en = np.zeros(self.nt, np.float32)
+        _check(self._lib.mc_pass(self._h, set_id, _f32(prop), _f32(uu), _u8(acc), _f32(eo), _f32(en)))
+        return acc, eo, en
+
+    def record_sweep(self):
+        nt = self.nt
+        r = {k: np.zeros(nt, np.float32) for k in ("u0", "u1", "u2", "e_old", "e_new")}
+        r["prop"] = np.zeros(3 * nt, np.float32)
+        r["accept"] = np.zeros(nt, np.uint8)
+        _check(self._lib.mc_record_sweep(self._h, _f32(r["u0"]), _f32(r["u1"]), _f32(r["u2"]), _f32(r["prop"]),
+                                         _u8(r["accept"]), _f32(r["e_old"]), _f32(r["e_new"])))
+        return r
+
+
+def device_count() -> int:
+    return int(load_library().mc_device_count())

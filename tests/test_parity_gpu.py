@@ -1,0 +1,254 @@
+"""GPU parity: the HIP engine against the CPU oracle through the C ABI.
+
+Injected proposals and uniforms (mc_pass) and recorded production sweeps
+(mc_record_sweep, replayed pass by pass in the oracle) must give bit-identical
+accept decisions, site energies and lattice state.  Energies are compared
+bit-exactly, which is stricter than the 1e-5 relative bound required.
+"""
+import math
+import zlib
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import mclics
+
+pytestmark = pytest.mark.gpu
+
+
+def unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32).reshape(-1)
+
+
+def in_set(nx, ny, nz, set_id):
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).reshape(-1) for a in (i, j, k))
+    if set_id < 8:
+        return (i % 2 == set_id % 2) & (j % 2 == (set_id // 2) % 2) & (k % 2 == set_id // 4)
+    return (i + j + k) % 2 == set_id - 8
+
+
+GHRL_K = (1.0, 0.7, 1.3, 16.0)
+
+
+def make_case(name, rng):
+    """(oracle params, engine config, pt, ns) for one geometry/physics case."""
+    surf = []
+    if name == "ll_bulk":
+        dims, bound, pot, kw = (6, 4, 8), (1, 1, 1), O.LL, {"A": 1.3}
+    elif name == "ghrl_bulk_field":
+        dims, bound, pot, kw = (8, 6, 4), (1, 1, 1), O.GHRL, {"k": GHRL_K, "elec": (0.3, -0.2, 1.1), "elecA": 0.8,
+                                                              "elecE": 0.7}
+    elif name == "pear_free":
+        dims, bound, pot, kw = (5, 7, 4), (0, 0, 0), O.PEAR, {"A": 1.0, "B1": 0.3, "Cc": 0.3}
+    elif name.startswith("slab_"):
+        kind = name[5:]
+        dims, bound, pot = (8, 6, 6), (1, 1, 0), O.GHRL
+        d = (0.6, 0.0, 0.8)
+        surf = [(kind, 4.0, d), ("homeotropic_ghrl" if kind != "homeotropic_ghrl" else "fg", 2.5, d)]
+        kw = {"k": (1.48, 1.0, 2.28, 0.0), "elec": (0.0, 0.0, 1.0), "elecA": 1.0, "elecE": 0.45}
+    elif name == "ll_holes":
+        dims, bound, pot, kw = (6, 6, 6), (1, 1, 1), O.LL, {}
+    else:
+        raise KeyError(name)
+    nx, ny, nz = dims
+    nt = nx * ny * nz
+    po = O.params(nx, ny, nz, bound=bound, potential=pot, surfaces=surf, **kw)
+    pt = np.ones(nt, np.int32)
+    ns = None
+    if name.startswith("slab_"):
+        ns = np.zeros(3 * nt, np.float32)
+        for j in range(ny):
+            for i in range(nx):
+                s0 = i + nx * j
+                s1 = i + nx * (j + ny * (nz - 1))
+                pt[s0], pt[s1] = 2, 3
+                ns[3 * s0 + 2], ns[3 * s1 + 2] = -1.0, 1.0
+    if name == "ll_holes":
+        pt[rng.random(nt) < 0.2] = 0
+    cfg = mclics.Config(nx, ny, nz, bound=bound, potential=pot, A=po.A, B1=po.B1, B2=po.B2, C=po.C,
+                        ghrl=O.ghrl_tuple(po), elec=(po.elecX, po.elecY, po.elecZ), elecA=po.elecA, elecE=po.elecE,
+                        surfaces=[mclics.Surface(k, w, d) for k, w, d in surf], T=0.9, seed=7)
+    return po, cfg, pt, ns
+
+
+CASES = ["ll_bulk", "ghrl_bulk_field", "pear_free", "ll_holes"] + [
+    "slab_" + k for k in ("homeotropic", "rp", "fg", "strong", "homeotropic_ghrl", "rp_ghrl", "fg_ghrl",
+                          "strong_ghrl")]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_injected_pass_parity(case):
+    rng = np.random.default_rng(zlib.crc32(case.encode()))
+    po, cfg, pt, ns = make_case(case, rng)
+    nx, ny, nz = po.nx, po.ny, po.nz
+    nt = nx * ny * nz
+    n = unit(rng, nt)
+    n[np.repeat(pt == 0, 3)] = np.tile([0.0, 0.0, 1.0], int((pt == 0).sum())).astype(np.float32)
+    ref = n.copy()
+    with mclics.Lattice(cfg, n, pt, ns) as lat:
+        for rep in range(2):
+            for set_id in range(10):
+                # a mix of small rotations and fresh directions, and a spread of T
+                prop = unit(rng, nt)
+                small = rng.random(nt) < 0.5
+                jitter = (ref.reshape(-1, 3) + 0.2 * rng.normal(size=(nt, 3))).astype(np.float64)
+                jitter /= np.linalg.norm(jitter, axis=1, keepdims=True)
+                prop.reshape(-1, 3)[small] = jitter[small].astype(np.float32)
+                u = rng.random(nt).astype(np.float32)
+                T = float(np.float32([0.05, 0.9, 3.0][(set_id + rep) % 3]))
+                lat.set_T(T)
+                acc, eo, en = lat.pass_inject(set_id, prop, u)
+                _, racc, reo, ren = O.pass_inject(po, ref, pt, ns, set_id, np.float32(T), prop, u)
+                sel = in_set(nx, ny, nz, set_id) & (pt != 0)
+                assert np.array_equal(acc[sel], racc[sel]), (case, set_id)
+                assert np.array_equal(eo[sel], reo[sel]), (case, set_id, np.max(np.abs(eo[sel] - reo[sel])))
+                assert np.array_equal(en[sel], ren[sel]), (case, set_id)
+                assert np.array_equal(lat.directors(), ref), (case, set_id)
+        np.testing.assert_array_equal(lat.site_energies()[pt != 0], O.site_energies(po, ref, pt, ns)[pt != 0])
+
+
+@pytest.mark.parametrize("case,order", [("ll_bulk", mclics.ORDER_CHECKER), ("ll_bulk", mclics.ORDER_REF8),
+                                        ("ghrl_bulk_field", mclics.ORDER_CHECKER),
+                                        ("slab_homeotropic_ghrl", mclics.ORDER_REF8),
+                                        ("pear_free", mclics.ORDER_CHECKER), ("ll_holes", mclics.ORDER_REF8)])
+def test_recorded_sweep_replay(case, order):
+    rng = np.random.default_rng(11)
+    po, cfg, pt, ns = make_case(case, rng)
+    cfg.order = order
+    nx, ny, nz = po.nx, po.ny, po.nz
+    nt = nx * ny * nz
+    n = unit(rng, nt)
+    n[np.repeat(pt == 0, 3)] = np.tile([0.0, 0.0, 1.0], int((pt == 0).sum())).astype(np.float32)
+    ref = n.copy()
+    sets = list(range(8)) if order == mclics.ORDER_REF8 else [8, 9]
+    with mclics.Lattice(cfg, n, pt, ns) as lat:
+        for w in range(4):
+            av = lat.ang_var()
+            rec = lat.record_sweep()
+            total = 0
+            for set_id in sets:
+                cnt, racc, reo, ren = O.pass_inject(po, ref, pt, ns, set_id, np.float32(cfg.T), rec["prop"],
+                                                    rec["u2"])
+                sel = in_set(nx, ny, nz, set_id) & (pt != 0)
+                assert np.array_equal(rec["accept"][sel], racc[sel]), (case, w, set_id)
+                assert np.array_equal(rec["e_old"][sel], reo[sel]), (case, w, set_id)
+                assert np.array_equal(rec["e_new"][sel], ren[sel]), (case, w, set_id)
+                total += cnt
+            assert np.array_equal(lat.directors(), ref), (case, w)
+            acc, sweeps = lat.acceptance()
+            assert acc == total and sweeps == w + 1
+            assert lat.ang_var() == O.lib().orc_adapt_ang_var(av, total, int((pt != 0).sum()))
+
+
+def test_proposals_are_normalised_rotations():
+    # Each proposal is the current director rotated by pi*va (va = (2u0-1)*ang_var)
+    # about x, y or z chosen by u1 against 0.333/0.666, then renormalised
+    # (src/evolve.cu:143-163).  Compared with an FP64 rotation.
+    rng = np.random.default_rng(12)
+    nx = ny = nz = 12
+    nt = nx * ny * nz
+    cfg = mclics.Config(nx, ny, nz, T=1.1, seed=99)
+    n = unit(rng, nt)
+    with mclics.Lattice(cfg, n) as lat:
+        lat.set_ang_var(0.8)
+        old = lat.directors().reshape(-1, 3).astype(np.float64)
+        rec = lat.record_sweep()
+    u0, u1 = rec["u0"].astype(np.float64), rec["u1"].astype(np.float64)
+    prop = rec["prop"].reshape(-1, 3).astype(np.float64)
+    black = in_set(nx, ny, nz, 9)
+    # black sites see red neighbours already updated; their own director is unchanged before the pass
+    va = (2 * u0 - 1) * np.float32(0.8)
+    a = np.pi * va
+    c, s = np.cos(a), np.sin(a)
+    x, y, z = old[:, 0], old[:, 1], old[:, 2]
+    want = np.where((u1 < 0.333)[:, None], np.stack([x, y * c + z * s, z * c - y * s], 1),
+                    np.where((u1 < 0.666)[:, None], np.stack([x * c - z * s, y, z * c + x * s], 1),
+                             np.stack([x * c + y * s, y * c - x * s, z], 1)))
+    want /= np.linalg.norm(want, axis=1, keepdims=True)
+    err = np.abs(prop - want).max(1)
+    assert err.max() < 5e-6, err.max()
+    np.testing.assert_allclose(np.linalg.norm(prop, axis=1), 1.0, atol=3e-7)
+    # axis frequencies follow the 0.333/0.333/0.334 thresholds
+    frac = [(u1 < 0.333).mean(), ((u1 >= 0.333) & (u1 < 0.666)).mean()]
+    assert abs(frac[0] - 0.333) < 0.03 and abs(frac[1] - 0.333) < 0.03
+    assert black.sum() > 0
+
+
+def test_measure_matches_oracle():
+    rng = np.random.default_rng(13)
+    for case in ("ll_bulk", "ghrl_bulk_field", "slab_rp_ghrl", "ll_holes"):
+        po, cfg, pt, ns = make_case(case, rng)
+        nt = po.nx * po.ny * po.nz
+        n = unit(rng, nt)
+        n[np.repeat(pt == 0, 3)] = np.tile([0.0, 0.0, 1.0], int((pt == 0).sum())).astype(np.float32)
+        with mclics.Lattice(cfg, n, pt, ns) as lat:
+            lat.sweeps(3)
+            d = lat.directors()
+            E, S, q = lat.measure()
+        assert E == pytest.approx(O.energy_mean(po, d, pt, ns, gpu=1), rel=1e-12, abs=1e-12)
+        qo = O.qtensor(d, pt)
+        np.testing.assert_allclose(q, qo, rtol=1e-11, atol=1e-13)
+        Q = np.array([[qo[0], qo[1], qo[2]], [qo[1], qo[3], qo[4]], [qo[2], qo[4], qo[5]]])
+        assert S == pytest.approx(np.linalg.eigvalsh(Q)[-1], abs=1e-12)
+
+
+def test_graph_replay_matches_eager():
+    rng = np.random.default_rng(14)
+    nx, ny, nz = 16, 12, 10
+    nt = nx * ny * nz
+    n = unit(rng, nt)
+    cfg = mclics.Config(nx, ny, nz, T=0.95, seed=21)
+    with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
+        for rep in range(3):
+            a.sweeps(5)
+            b.sweeps(5, graph=True)
+            a.sync()
+            b.sync()
+            # eager work between replays
+            assert np.array_equal(a.directors(), b.directors()), rep
+            assert a.measure()[0] == b.measure()[0]
+            assert a.ang_var() == b.ang_var()
+            assert a.acceptance() == b.acceptance()
+
+
+def test_full_size_256_properties():
+    # The headline lattice: 256^3 LL, random start (seed 2).  Size-independent
+    # properties: unit norms kept, counters, E inside [-9A, 0], S in [0, 1],
+    # ordering at T=0.6 lowers E, measurement agrees with a host recount of Q.
+    L = 256
+    n = mclics.random_ic(L, L, L, seed=2)
+    cfg = mclics.Config(L, L, L, T=0.6, seed=2)
+    with mclics.Lattice(cfg, n) as lat:
+        E0, S0, _ = lat.measure()
+        lat.sweeps(20, graph=True)
+        E1, S1, q = lat.measure()
+        acc, sweeps = lat.acceptance()
+        d = lat.directors().reshape(-1, 3)
+    assert sweeps == 20 and 0 < acc < L**3
+    norms = np.linalg.norm(d.astype(np.float64), axis=1)
+    assert np.abs(norms - 1).max() < 2e-6
+    assert -9.0 <= E1 < E0 < -2.9
+    assert 0.0 <= S0 < 0.01 and S0 < S1 <= 1.0
+    # Q from the host in FP64 (chunked to bound memory)
+    acc6 = np.zeros(6)
+    for a in range(0, len(d), 1 << 22):
+        x = d[a:a + (1 << 22)].astype(np.float64)
+        acc6 += [np.sum(3 * x[:, 0] ** 2 - 1), np.sum(3 * x[:, 0] * x[:, 1]), np.sum(3 * x[:, 0] * x[:, 2]),
+                 np.sum(3 * x[:, 1] ** 2 - 1), np.sum(3 * x[:, 1] * x[:, 2]), np.sum(3 * x[:, 2] ** 2 - 1)]
+    np.testing.assert_allclose(q, acc6 / (2 * len(d)), rtol=1e-9, atol=1e-12)
+
+
+def test_create_rejects_bad_input():
+    n = unit(np.random.default_rng(0), 64)
+    with pytest.raises(mclics.EngineError):
+        mclics.Lattice(mclics.Config(4, 4, 4), n * 2)  # non-unit directors
+    with pytest.raises(mclics.EngineError):
+        mclics.Lattice(mclics.Config(5, 4, 4), unit(np.random.default_rng(0), 80))  # odd periodic axis
+    with mclics.Lattice(mclics.Config(4, 4, 4), n) as lat:
+        with pytest.raises(mclics.EngineError):
+            lat.set_T(-1.0)
+        with pytest.raises(mclics.EngineError):
+            lat.pass_inject(11, n, np.zeros(64, np.float32))
