@@ -115,6 +115,11 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
  * FP32 site energies over active sites divided by their count; q6 (may be
  * NULL) receives {Qxx,Qxy,Qxz,Qyy,Qyz,Qzz}. */
 int mc_measure(mc_handle *h, double *E, double *S, double *q6);
+/* n times (one sweep, then the measurement above), queued without host
+ * synchronisation; E_series/S_series (double[n], either may be NULL) receive
+ * the per-sweep values.  This is the MCS loop of the mode drivers
+ * (src/evolve_thermal.cu:85-95) with the lattice never leaving the device. */
+int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series);
 /* FP32 site energies of every site (0 at pt==0 sites), float[Nt]. */
 int mc_site_energies(mc_handle *h, float *out);
 

@@ -72,6 +72,8 @@ struct mc_handle {
   MCDevState *st = nullptr;
   double *partials = nullptr;
   double *result = nullptr;
+  double *series = nullptr;  // per-step {E, S} of mc_measure_series
+  int series_cap = 0;
   float *staging = nullptr;  // AoS float[3*Nt]
   int measure_grid = 0;
   // Parity and record buffers, allocated on first use.
@@ -110,7 +112,7 @@ void free_handle(mc_handle *h) {
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   void *ptrs[] = {h->nx, h->ny, h->nz, h->pt, h->sx, h->sy, h->sz, h->st, h->partials, h->result, h->staging,
-                  h->pbuf, h->abuf};
+                  h->pbuf, h->abuf, h->series};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -485,7 +487,7 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
 int mc_measure(mc_handle *h, double *E, double *S, double *q6) {
   int rc = check_handle(h);
   if (rc) return rc;
-  mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result);
+  mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result, nullptr);
   MC_LAUNCHED("measure");
   double r[9];
   MC_HIP(hipMemcpyAsync(r, h->result, sizeof(r), hipMemcpyDeviceToHost, h->stream));
@@ -494,6 +496,34 @@ int mc_measure(mc_handle *h, double *E, double *S, double *q6) {
   if (S) *S = r[1];
   if (q6)
     for (int c = 0; c < 6; ++c) q6[c] = r[2 + c];
+  return MC_OK;
+}
+
+int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (n < 1) return fail(MC_EINVAL, "mc_measure_series: need n >= 1");
+  if (n > h->series_cap) {
+    if (h->series) MC_HIP(hipFree(h->series));
+    h->series = nullptr;
+    h->series_cap = 0;
+    MC_HIP(hipMalloc(&h->series, 2 * (size_t)n * sizeof(double)));
+    h->series_cap = n;
+  }
+  const MCLaunch L = h->launch();
+  for (int w = 0; w < n; ++w) {
+    rc = enqueue_sweep(h, nullptr);
+    if (rc) return rc;
+    mc_launch_measure(L, h->measure_grid, h->partials, h->result, h->series + 2 * (size_t)w);
+  }
+  MC_LAUNCHED("measured sweep");
+  std::vector<double> buf(2 * (size_t)n);
+  MC_HIP(hipMemcpyAsync(buf.data(), h->series, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  for (int w = 0; w < n; ++w) {
+    if (E_series) E_series[w] = buf[2 * (size_t)w];
+    if (S_series) S_series[w] = buf[2 * (size_t)w + 1];
+  }
   return MC_OK;
 }
 

@@ -28,7 +28,8 @@ void mc_launch_sweep_pass(const MCLaunch &L, int set, int grid, int finalize, co
 void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *prop, const float *u, uint8_t *acc,
                            float *eo, float *en);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
-void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out);
+// out = {E, S, q6, n_active}; series (may be null) receives {E, S}.
+void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series);
 void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, float *z, int n, int grid);
 void mc_launch_soa_to_aos(hipStream_t s, const float *x, const float *y, const float *z, float *aos, int n,
                           int grid);

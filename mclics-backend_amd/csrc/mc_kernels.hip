@@ -339,7 +339,7 @@ __device__ double mcd_sym3_max(const double q[6]) {
 // Second stage, one workgroup, fixed summation order (deterministic):
 // out = {E_mean, S, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n_active}.
 __global__ __launch_bounds__(kBlock) void k_measure_final(const double *__restrict__ partials, int nparts,
-                                                          double *__restrict__ out) {
+                                                          double *__restrict__ out, double *__restrict__ series) {
   __shared__ double red[8][kBlock];
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int p = threadIdx.x; p < nparts; p += kBlock)
@@ -362,6 +362,10 @@ __global__ __launch_bounds__(kBlock) void k_measure_final(const double *__restri
     out[1] = mcd_sym3_max(q);
     for (int c = 0; c < 6; ++c) out[2 + c] = q[c];
     out[8] = n;
+    if (series) {
+      series[0] = out[0];
+      series[1] = out[1];
+    }
   }
 }
 
@@ -450,9 +454,9 @@ void mc_launch_site_energy(const MCLaunch &L, int grid, float *out) {
   MC_DISPATCH(k_site_energy, grid, L.P, L.nx, L.ny, L.nz, L.pt, L.sx, L.sy, L.sz, L.st, out);
 }
 
-void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out) {
+void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series) {
   MC_DISPATCH(k_measure, grid, L.P, L.nx, L.ny, L.nz, L.pt, L.sx, L.sy, L.sz, L.st, partials);
-  k_measure_final<<<1, kBlock, 0, L.stream>>>(partials, grid, out);
+  k_measure_final<<<1, kBlock, 0, L.stream>>>(partials, grid, out, series);
 }
 
 void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, float *z, int n, int grid) {

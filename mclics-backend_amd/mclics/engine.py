@@ -47,6 +47,7 @@ EXPORTS = (
     "mc_time_sweeps",
     "mc_time_passes",
     "mc_measure",
+    "mc_measure_series",
     "mc_site_energies",
     "mc_get_directors_aos",
     "mc_set_directors_aos",
@@ -126,6 +127,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_time_sweeps.argtypes = [vp, C.c_int, C.c_int, f32p, C.POINTER(C.c_int)]
     lib.mc_time_passes.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mc_measure_series.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_site_energies.argtypes = [vp, f32p]
     lib.mc_get_directors_aos.argtypes = [vp, f32p]
     lib.mc_set_directors_aos.argtypes = [vp, f32p]
@@ -297,6 +299,13 @@ class Lattice:
         q = (C.c_double * 6)()
         _check(self._lib.mc_measure(self._h, C.byref(E), C.byref(S), q))
         return E.value, S.value, np.array(q[:], np.float64)
+
+    def measure_series(self, n: int):
+        E = np.empty(n, np.float64)
+        S = np.empty(n, np.float64)
+        _check(self._lib.mc_measure_series(self._h, n, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                           S.ctypes.data_as(C.POINTER(C.c_double))))
+        return E, S
 
     def site_energies(self) -> np.ndarray:
         out = np.empty(self.nt, np.float32)
