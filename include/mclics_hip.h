@@ -48,6 +48,12 @@ enum mc_anchoring {
  * (src/evolve.cpp:100-102); MC_ORDER_CHECKER updates the two (i+j+k) parity
  * colours, a different but equally valid Markov chain. */
 enum mc_order { MC_ORDER_CHECKER = 0, MC_ORDER_REF8 = 1 };
+/* Sweep kernels.  AUTO picks the segment kernel (X4) whenever the lattice is
+ * all active, periodic on every axis, has Nx % 4 == 0 and uses the
+ * checkerboard order, else the generic site kernel.  The two use different
+ * counter layouts of the generator, so they are different (equally valid)
+ * chains from the same seed. */
+enum mc_kernel { MC_KERNEL_AUTO = 0, MC_KERNEL_GENERIC = 1, MC_KERNEL_X4 = 2 };
 
 /* POD flattening of the reference's Parameters (include/parameters.h:11-83)
  * restricted to what the sweep reads.  The std::map surface tables become
@@ -91,6 +97,10 @@ int mc_set_T(mc_handle *h, float T);
 int mc_set_field(mc_handle *h, float elecE);
 int mc_set_ang_var(mc_handle *h, float ang_var);
 int mc_get_ang_var(mc_handle *h, float *ang_var);
+
+/* Sweep kernel selection (mc_kernel); get returns the kernel in use. */
+int mc_set_sweep_kernel(mc_handle *h, int kind);
+int mc_get_sweep_kernel(mc_handle *h, int *kind);
 
 /* n full sweeps (EvolveNGPU::Monte_Carlo_Step_GPU, src/evolve.cu:265-388),
  * asynchronous on the handle's stream, with the ang_var adaptation done on

@@ -65,6 +65,7 @@ struct mc_handle {
   hipStream_t stream = nullptr;
   MCDevParams P{};
   int order = MC_ORDER_CHECKER;
+  int kernel = MC_KERNEL_AUTO;  // sweep kernel selection
   int cu_count = 256;
   float *nx = nullptr, *ny = nullptr, *nz = nullptr;
   uint8_t *pt = nullptr;
@@ -127,15 +128,28 @@ int check_handle(mc_handle *h) {
   return MC_OK;
 }
 
-int enqueue_sweep(mc_handle *h, const MCRecord *rec) {
+bool use_x4(const mc_handle *h) {
+  if (h->order != MC_ORDER_CHECKER || h->kernel == MC_KERNEL_GENERIC) return false;
+  return mc_x4_eligible(h->launch());
+}
+
+int passes_per_sweep(const mc_handle *h) { return h->order == MC_ORDER_REF8 ? 8 : 2; }
+
+// Pass t (0-based) of one sweep; the last pass adapts ang_var.
+void launch_pass(mc_handle *h, int t, const MCRecord *rec) {
   const MCLaunch L = h->launch();
-  if (h->order == MC_ORDER_REF8) {
-    for (int tick = 0; tick < 8; ++tick)
-      mc_launch_sweep_pass(L, tick, h->grid_for(mc_set_slots(h->P, tick)), tick == 7, rec);
+  const int per = passes_per_sweep(h);
+  const int finalize = t == per - 1;
+  if (use_x4(h)) {
+    mc_launch_sweep_x4(L, t, h->grid_for(mc_x4_slots(h->P)), finalize, rec);
   } else {
-    mc_launch_sweep_pass(L, 8, h->grid_for(mc_set_slots(h->P, 8)), 0, rec);
-    mc_launch_sweep_pass(L, 9, h->grid_for(mc_set_slots(h->P, 9)), 1, rec);
+    const int set = per == 8 ? t : 8 + t;
+    mc_launch_sweep_pass(L, set, h->grid_for(mc_set_slots(h->P, set)), finalize, rec);
   }
+}
+
+int enqueue_sweep(mc_handle *h, const MCRecord *rec) {
+  for (int t = 0; t < passes_per_sweep(h); ++t) launch_pass(h, t, rec);
   MC_LAUNCHED("sweep pass");
   return MC_OK;
 }
@@ -366,6 +380,29 @@ int mc_get_acceptance(mc_handle *h, uint32_t *accepted, uint32_t *sweeps_done) {
   return MC_OK;
 }
 
+int mc_set_sweep_kernel(mc_handle *h, int kind) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_GENERIC && kind != MC_KERNEL_X4)
+    return fail(MC_EINVAL, "mc_set_sweep_kernel: unknown kernel kind");
+  if (kind == MC_KERNEL_X4 && (h->order != MC_ORDER_CHECKER || !mc_x4_eligible(h->launch())))
+    return fail(MC_EINVAL, "mc_set_sweep_kernel: the segment kernel needs a checkerboard order and an all-active "
+                           "lattice, periodic on every axis, with Nx a multiple of 4");
+  MC_HIP(hipStreamSynchronize(h->stream));
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  h->kernel = kind;
+  return MC_OK;
+}
+
+int mc_get_sweep_kernel(mc_handle *h, int *kind) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!kind) return fail(MC_EINVAL, "mc_get_sweep_kernel: null output");
+  *kind = use_x4(h) ? MC_KERNEL_X4 : MC_KERNEL_GENERIC;
+  return MC_OK;
+}
+
 int mc_sweeps(mc_handle *h, int n) {
   int rc = check_handle(h);
   if (rc) return rc;
@@ -444,7 +481,7 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   MC_HIP(hipEventRecord(h->ev1, h->stream));
   MC_HIP(hipEventSynchronize(h->ev1));
   MC_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1));
-  if (launches) *launches = n * (h->order == MC_ORDER_REF8 ? 8 : 2);
+  if (launches) *launches = n * passes_per_sweep(h);
   return MC_OK;
 }
 
@@ -452,17 +489,15 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
   int rc = check_handle(h);
   if (rc) return rc;
   if (n < 1 || !avg_pass_ms) return fail(MC_EINVAL, "mc_time_passes: need n >= 1 and an output");
-  const int per = h->order == MC_ORDER_REF8 ? 8 : 2;
+  const int per = passes_per_sweep(h);
   const int np = n * per;
   std::vector<hipEvent_t> ev(2 * (size_t)np, nullptr);
   for (auto &e : ev) MC_HIP(hipEventCreate(&e));
-  const MCLaunch L = h->launch();
   MC_HIP(hipEventRecord(h->ev0, h->stream));
   for (int w = 0, q = 0; w < n; ++w) {
     for (int t = 0; t < per; ++t, ++q) {
-      const int set = per == 8 ? t : 8 + t;
       MC_HIP(hipEventRecord(ev[2 * q], h->stream));
-      mc_launch_sweep_pass(L, set, h->grid_for(mc_set_slots(h->P, set)), t == per - 1, nullptr);
+      launch_pass(h, t, nullptr);
       MC_HIP(hipEventRecord(ev[2 * q + 1], h->stream));
     }
   }

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #define MCD_MAX_SURF 8
+#define MCD_BLOCK 256  // threads per workgroup of every sweep kernel
 
 enum { MCD_LL = 0, MCD_GHRL = 1, MCD_PEAR = 2 };
 enum {
@@ -240,12 +241,57 @@ __device__ __forceinline__ float mcd_bond(const MCDevParams &P, double cA, int b
 
 // Accept rule of the reference CPU sweep, u < expf(-dE/T) (src/evolve.cpp:160).
 // A hardware exp2 decides every case whose margin exceeds its error bound
-// (|x| <= 2^11 keeps the relative error of exp2(x*log2e) under 2^-13); the
+// (|x| <= 2^11 keeps the relative error of exp2(x*log2e) under 2^-12); the
 // rare near-ties, u == 0 and infinities are settled with an FP64 exp rounded
 // to FP32, i.e. the correctly rounded expf the CPU returns.
 __device__ __forceinline__ bool mcd_accept(float E_new, float E_old, float T, float u) {
   float x = -(E_new - E_old) / T;
   float pf = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
-  if (u != 0.0f && !(fabsf(u - pf) <= pf * 1.220703125e-4f)) return u < pf;
+  if (u != 0.0f && !(fabsf(u - pf) <= pf * 2.44140625e-4f)) return u < pf;
   return u < (float)exp((double)x);
 }
+
+// ---------------------------------------------------------------------------
+// Acceptance bookkeeping shared by the sweep kernels.
+
+// Wave64 + workgroup sum of a per-thread count.
+__device__ __forceinline__ uint32_t mcd_block_sum(uint32_t v) {
+  __shared__ uint32_t wsum[MCD_BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < MCD_BLOCK / 64; ++w) t += wsum[w];
+  return t;
+}
+
+// Device-side ang_var adaptation by the last workgroup of a sweep's final
+// pass, src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
+__device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, uint32_t block_acc, bool finalize) {
+  if (threadIdx.x != 0) return;
+  atomicAdd(&st->acc_count, block_acc);
+  if (!finalize) return;
+  __threadfence();
+  const uint32_t done = atomicAdd(&st->blocks_done, 1u);
+  if (done != gridDim.x - 1) return;
+  __threadfence();
+  const uint32_t acc = atomicAdd(&st->acc_count, 0u);
+  float a = st->ang_var;
+  const double rate = 1.0 * (double)acc / (double)P.valid;
+  if (rate < 0.5) {
+    a = (float)((double)a * 0.99);
+    if ((double)a < 0.01) a = (float)0.01;
+  } else if (rate > 0.5) {
+    a = (float)((double)a / 0.99);
+    if ((double)a > 1.0) a = (float)((double)a - 0.5);
+  }
+  st->ang_var = a;
+  st->last_acc = acc;
+  st->acc_count = 0u;
+  st->blocks_done = 0u;
+  st->sweep = st->sweep + 1u;
+}
+

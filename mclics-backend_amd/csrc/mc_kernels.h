@@ -27,6 +27,10 @@ int mc_set_slots(const MCDevParams &P, int set);
 void mc_launch_sweep_pass(const MCLaunch &L, int set, int grid, int finalize, const MCRecord *rec);
 void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *prop, const float *u, uint8_t *acc,
                            float *eo, float *en);
+// Fast pass for all-active periodic lattices with Nx % 4 == 0 (mc_sweep_x4.hip).
+bool mc_x4_eligible(const MCLaunch &L);
+int mc_x4_slots(const MCDevParams &P);
+void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, const MCRecord *rec);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
 // out = {E, S, q6, n_active}; series (may be null) receives {E, S}.
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series);

@@ -16,7 +16,7 @@
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = MCD_BLOCK;
 
 // Maps a linear slot of a site set to (i, j, k); returns false for slots
 // outside the lattice.
@@ -103,47 +103,6 @@ __device__ __forceinline__ void mcd_site_energy(const MCDevParams &P, double cA,
     Eo += mcd_field(P, elecE, o0, o1, o2);
     if (TWO) En += mcd_field(P, elecE, q0, q1, q2);
   }
-}
-
-// Wave64 + workgroup sum of a per-thread count.
-__device__ __forceinline__ uint32_t mcd_block_sum(uint32_t v) {
-  __shared__ uint32_t wsum[kBlock / 64];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) wsum[wid] = v;
-  __syncthreads();
-  uint32_t t = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
-  return t;
-}
-
-// Device-side ang_var adaptation by the last workgroup of a sweep's final
-// pass, src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
-__device__ void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, uint32_t block_acc, bool finalize) {
-  if (threadIdx.x != 0) return;
-  atomicAdd(&st->acc_count, block_acc);
-  if (!finalize) return;
-  __threadfence();
-  const uint32_t done = atomicAdd(&st->blocks_done, 1u);
-  if (done != gridDim.x - 1) return;
-  __threadfence();
-  const uint32_t acc = atomicAdd(&st->acc_count, 0u);
-  float a = st->ang_var;
-  const double rate = 1.0 * (double)acc / (double)P.valid;
-  if (rate < 0.5) {
-    a = (float)((double)a * 0.99);
-    if ((double)a < 0.01) a = (float)0.01;
-  } else if (rate > 0.5) {
-    a = (float)((double)a / 0.99);
-    if ((double)a > 1.0) a = (float)((double)a - 0.5);
-  }
-  st->ang_var = a;
-  st->last_acc = acc;
-  st->acc_count = 0u;
-  st->blocks_done = 0u;
-  st->sweep = st->sweep + 1u;
 }
 
 // Production pass: counter-based proposal, fused old/new energy, accept.

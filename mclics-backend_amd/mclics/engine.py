@@ -20,6 +20,7 @@ MC_MAX_SURF = 8
 FREE, PERIODIC = 0, 1
 LL, GHRL, PEAR = 0, 1, 2
 ORDER_CHECKER, ORDER_REF8 = 0, 1
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_X4 = 0, 1, 2
 ANCHORING = {
     "homeotropic": 0,
     "rp": 1,
@@ -41,6 +42,8 @@ EXPORTS = (
     "mc_set_field",
     "mc_set_ang_var",
     "mc_get_ang_var",
+    "mc_set_sweep_kernel",
+    "mc_get_sweep_kernel",
     "mc_sweeps",
     "mc_sweeps_graph",
     "mc_sync",
@@ -121,6 +124,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_set_field.argtypes = [vp, C.c_float]
     lib.mc_set_ang_var.argtypes = [vp, C.c_float]
     lib.mc_get_ang_var.argtypes = [vp, f32p]
+    lib.mc_set_sweep_kernel.argtypes = [vp, C.c_int]
+    lib.mc_get_sweep_kernel.argtypes = [vp, C.POINTER(C.c_int)]
     lib.mc_sweeps.argtypes = [vp, C.c_int]
     lib.mc_sweeps_graph.argtypes = [vp, C.c_int]
     lib.mc_sync.argtypes = [vp]
@@ -276,6 +281,14 @@ class Lattice:
     def set_directors(self, ni_aos: np.ndarray) -> None:
         ni = np.ascontiguousarray(ni_aos, dtype=np.float32).reshape(3 * self.nt)
         _check(self._lib.mc_set_directors_aos(self._h, _f32(ni)))
+
+    def set_kernel(self, kind: int) -> None:
+        _check(self._lib.mc_set_sweep_kernel(self._h, kind))
+
+    def kernel(self) -> int:
+        v = C.c_int()
+        _check(self._lib.mc_get_sweep_kernel(self._h, C.byref(v)))
+        return v.value
 
     # --- work --------------------------------------------------------------
     def sweeps(self, n: int, graph: bool = False) -> None:

@@ -125,6 +125,7 @@ def test_recorded_sweep_replay(case, order):
     ref = n.copy()
     sets = list(range(8)) if order == mclics.ORDER_REF8 else [8, 9]
     with mclics.Lattice(cfg, n, pt, ns) as lat:
+        lat.set_kernel(mclics.KERNEL_GENERIC)
         for w in range(4):
             av = lat.ang_var()
             rec = lat.record_sweep()
@@ -153,6 +154,7 @@ def test_proposals_are_normalised_rotations():
     cfg = mclics.Config(nx, ny, nz, T=1.1, seed=99)
     n = unit(rng, nt)
     with mclics.Lattice(cfg, n) as lat:
+        lat.set_kernel(mclics.KERNEL_GENERIC)
         lat.set_ang_var(0.8)
         old = lat.directors().reshape(-1, 3).astype(np.float64)
         rec = lat.record_sweep()
@@ -252,3 +254,77 @@ def test_create_rejects_bad_input():
             lat.set_T(-1.0)
         with pytest.raises(mclics.EngineError):
             lat.pass_inject(11, n, np.zeros(64, np.float32))
+
+
+@pytest.mark.parametrize("pot,T,field", [(O.LL, 1.13, False), (O.LL, 0.6, True), (O.LL, 1e-4, False),
+                                         (O.LL, 40.0, False), (O.GHRL, 0.5, True), (O.PEAR, 0.8, False)])
+def test_segment_kernel_replay(pot, T, field):
+    # The x4 segment kernel (fast FP32 screen + exact fallback for LL) must
+    # make exactly the CPU path's decisions: replay recorded sweeps in the
+    # oracle.  T = 1e-4 sends nearly every move through the exact fallback,
+    # T = 40 makes most of them fast accepts.
+    rng = np.random.default_rng(15)
+    nx, ny, nz = 16, 8, 12
+    nt = nx * ny * nz
+    kw = {}
+    if pot == O.GHRL:
+        kw["k"] = GHRL_K
+    if pot == O.PEAR:
+        kw.update(A=1.0, B1=0.3, Cc=0.3)
+    if field:
+        kw.update(elec=(0.2, 0.1, 1.0), elecA=0.9, elecE=0.8)
+    po = O.params(nx, ny, nz, potential=pot, **kw)
+    cfg = mclics.Config(nx, ny, nz, potential=pot, A=po.A, B1=po.B1, C=po.C, ghrl=O.ghrl_tuple(po),
+                        elec=(po.elecX, po.elecY, po.elecZ), elecA=po.elecA, elecE=po.elecE,
+                        T=float(np.float32(T)), seed=31)
+    n = unit(rng, nt)
+    ref = n.copy()
+    pt = np.ones(nt, np.int32)
+    with mclics.Lattice(cfg, n) as lat:
+        lat.set_kernel(mclics.KERNEL_X4)
+        assert lat.kernel() == mclics.KERNEL_X4
+        for w in range(3):
+            av = lat.ang_var()
+            pre = ref.copy()
+            rec = lat.record_sweep()
+            total = 0
+            for colour in (0, 1):
+                cnt, racc, reo, ren = O.pass_inject(po, ref, pt, None, 8 + colour, np.float32(T), rec["prop"],
+                                                    rec["u2"])
+                sel = in_set(nx, ny, nz, 8 + colour)
+                assert np.array_equal(rec["accept"][sel], racc[sel]), (w, colour)
+                assert np.array_equal(rec["e_old"][sel], reo[sel]), (w, colour)
+                assert np.array_equal(rec["e_new"][sel], ren[sel]), (w, colour)
+                total += cnt
+            assert np.array_equal(lat.directors(), ref), w
+            assert lat.acceptance() == (total, w + 1)
+            assert lat.ang_var() == O.lib().orc_adapt_ang_var(av, total, nt)
+    # the proposals are normalised rotations by pi*va about the chosen axis
+    u0, u1 = rec["u0"].astype(np.float64), rec["u1"].astype(np.float64)
+    assert ((u1 >= 0) & (u1 < 1)).all() and ((u0 >= 0) & (u0 < 1)).all()
+    old = pre.reshape(-1, 3).astype(np.float64)
+    a = np.pi * (2 * u0 - 1) * np.float64(np.float32(av))
+    c, s_ = np.cos(a), np.sin(a)
+    x, y, z = old[:, 0], old[:, 1], old[:, 2]
+    want = np.where((u1 < 0.333)[:, None], np.stack([x, y * c + z * s_, z * c - y * s_], 1),
+                    np.where((u1 < 0.666)[:, None], np.stack([x * c - z * s_, y, z * c + x * s_], 1),
+                             np.stack([x * c + y * s_, y * c - x * s_, z], 1)))
+    want /= np.linalg.norm(want, axis=1, keepdims=True)
+    assert np.abs(rec["prop"].reshape(-1, 3) - want).max() < 5e-6
+    np.testing.assert_allclose(np.linalg.norm(rec["prop"].reshape(-1, 3), axis=1), 1.0, atol=3e-7)
+
+
+def test_segment_kernel_selection_rules():
+    rng = np.random.default_rng(16)
+    n = unit(rng, 6 * 4 * 4)
+    with mclics.Lattice(mclics.Config(6, 4, 4), n) as lat:  # Nx % 4 != 0
+        assert lat.kernel() == mclics.KERNEL_GENERIC
+        with pytest.raises(mclics.EngineError):
+            lat.set_kernel(mclics.KERNEL_X4)
+    n = unit(rng, 8 * 4 * 4)
+    with mclics.Lattice(mclics.Config(8, 4, 4), n) as lat:
+        assert lat.kernel() == mclics.KERNEL_X4
+        lat.set_kernel(mclics.KERNEL_GENERIC)
+        assert lat.kernel() == mclics.KERNEL_GENERIC
+    with mclics.Lattice(mclics.Config(8, 4, 4, order=mclics.ORDER_REF8), n) as lat:
+        assert lat.kernel() == mclics.KERNEL_GENERIC
