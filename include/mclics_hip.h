@@ -102,6 +102,15 @@ int mc_get_ang_var(mc_handle *h, float *ang_var);
 int mc_set_sweep_kernel(mc_handle *h, int kind);
 int mc_get_sweep_kernel(mc_handle *h, int *kind);
 
+/* Launch shape of the segment kernel: grid = number of persistent
+ * workgroups (0 = 8 per CU), xcd = 1 gives each of the 8 XCDs a contiguous
+ * eighth of the tiles (grid must then be a multiple of 8). */
+int mc_set_tuning(mc_handle *h, int grid, int xcd);
+/* Times the candidate launch shapes on a scratch copy of the lattice (the
+ * chain itself is not advanced), keeps the fastest and reports it with its
+ * mean sweep time in microseconds (any output may be NULL). */
+int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us);
+
 /* n full sweeps (EvolveNGPU::Monte_Carlo_Step_GPU, src/evolve.cu:265-388),
  * asynchronous on the handle's stream, with the ang_var adaptation done on
  * the device after each sweep so no host round trip is needed. */
@@ -154,6 +163,32 @@ int mc_pass(mc_handle *h, int set, const float *prop_aos, const float *u, uint8_
  * CPU oracle can replay it pass by pass. */
 int mc_record_sweep(mc_handle *h, float *u0, float *u1, float *u2, float *prop_aos, uint8_t *accept,
                     float *e_old, float *e_new);
+
+/* ------------------------------------------------------------------------
+ * Batched replicas.  nrep independent bulk lattices (periodic on every axis,
+ * Nx % 4 == 0, even Ny and Nz, one potential for all) of any sizes, each with
+ * its own parameters, temperature, seed/stream and proposal width, held in
+ * one device allocation.  A sweep of all replicas is two persistent launches
+ * (one per colour) over the tiles of every replica, replayed as one captured
+ * graph; every replica evolves bit-identically to the same lattice run alone
+ * with the segment kernel.  No reference counterpart: the reference runs one
+ * lattice per process (src/evolve.cu:265-270 keeps per-process statics). */
+typedef struct mc_batch mc_batch;
+int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_aos, mc_batch **out);
+int mc_batch_destroy(mc_batch *b);
+int mc_batch_set_T(mc_batch *b, int rep, float T);
+int mc_batch_sweeps(mc_batch *b, int n);
+int mc_batch_sync(mc_batch *b);
+int mc_batch_time_sweeps(mc_batch *b, int n, float *ms);
+/* Per-replica mean site energy and order parameter (double[nrep] each). */
+int mc_batch_measure(mc_batch *b, double *E, double *S);
+/* n measured sweeps; E/S are double[n*nrep], sweep-major. */
+int mc_batch_measure_series(mc_batch *b, int n, double *E, double *S);
+int mc_batch_get_directors_aos(mc_batch *b, int rep, float *ni_aos);
+/* Per-replica accepted count of the last sweep, sweep counter, ang_var. */
+int mc_batch_get_state(mc_batch *b, uint32_t *accepted, uint32_t *sweeps, float *ang_var);
+int mc_batch_set_tuning(mc_batch *b, int grid, int xcd);
+int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us);
 
 #ifdef __cplusplus
 }

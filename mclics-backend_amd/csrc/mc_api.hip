@@ -60,12 +60,16 @@ __global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var
 
 }  // namespace
 
+void mc_set_error_message(const char *msg) { g_err = msg; }
+
 struct mc_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   MCDevParams P{};
   int order = MC_ORDER_CHECKER;
   int kernel = MC_KERNEL_AUTO;  // sweep kernel selection
+  int x4_grid = 0;              // workgroups of a segment pass (0: 8 per CU)
+  int x4_xcd = 0;               // tile schedule of the segment pass
   int cu_count = 256;
   float *nx = nullptr, *ny = nullptr, *nz = nullptr;
   uint8_t *pt = nullptr;
@@ -141,7 +145,8 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec) {
   const int per = passes_per_sweep(h);
   const int finalize = t == per - 1;
   if (use_x4(h)) {
-    mc_launch_sweep_x4(L, t, h->grid_for(mc_x4_slots(h->P)), finalize, rec);
+    const int g = h->x4_grid > 0 ? h->x4_grid : h->cu_count * 8;
+    mc_launch_sweep_x4(L, t, g, finalize, h->x4_xcd, rec);
   } else {
     const int set = per == 8 ? t : 8 + t;
     mc_launch_sweep_pass(L, set, h->grid_for(mc_set_slots(h->P, set)), finalize, rec);
@@ -162,6 +167,38 @@ int ensure_parity_buffers(mc_handle *h) {
 }
 
 }  // namespace
+
+const char *mc_to_dev_params(const mc_params *p, MCDevParams *out) {
+  MCDevParams &P = *out;
+  P = MCDevParams{};
+  P.nx = p->nx;
+  P.ny = p->ny;
+  P.nz = p->nz;
+  for (int a = 0; a < 3; ++a) P.per[a] = p->bound[a] == MC_PERIODIC;
+  P.potential = p->potential;
+  P.A = p->A;
+  P.B1 = p->B1;
+  P.C = p->C;
+  P.rho = p->ghrl_rho;
+  P.lambda = p->ghrl_lambda;
+  P.mu = p->ghrl_mu;
+  P.nu = p->ghrl_nu;
+  P.sigma = p->ghrl_sigma;
+  P.eX = p->elecX;
+  P.eY = p->elecY;
+  P.eZ = p->elecZ;
+  P.elecA = p->elecA;
+  P.n_surf = p->n_surf;
+  for (int s = 0; s < MC_MAX_SURF; ++s) {
+    P.surf_kind[s] = s < p->n_surf ? p->surf_kind[s] : 0;
+    P.surf_W[s] = s < p->n_surf ? p->surf_W[s] : 0.0f;
+    for (int c = 0; c < 3; ++c) P.surf_dir[s][c] = s < p->n_surf ? p->surf_dir[s][c] : 0.0f;
+  }
+  P.seed_lo = (uint32_t)(p->seed & 0xFFFFFFFFu);
+  P.seed_hi = (uint32_t)(p->seed >> 32);
+  P.stream = p->stream;
+  return nullptr;
+}
 
 extern "C" {
 
@@ -258,35 +295,9 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   MC_HIPC(hipEventCreate(&h->ev0));
   MC_HIPC(hipEventCreate(&h->ev1));
 
-  MCDevParams &P = h->P;
-  P.nx = p->nx;
-  P.ny = p->ny;
-  P.nz = p->nz;
-  for (int a = 0; a < 3; ++a) P.per[a] = p->bound[a] == MC_PERIODIC;
-  P.potential = p->potential;
-  P.A = p->A;
-  P.B1 = p->B1;
-  P.C = p->C;
-  P.rho = p->ghrl_rho;
-  P.lambda = p->ghrl_lambda;
-  P.mu = p->ghrl_mu;
-  P.nu = p->ghrl_nu;
-  P.sigma = p->ghrl_sigma;
-  P.eX = p->elecX;
-  P.eY = p->elecY;
-  P.eZ = p->elecZ;
-  P.elecA = p->elecA;
-  P.n_surf = p->n_surf;
-  for (int s = 0; s < MC_MAX_SURF; ++s) {
-    P.surf_kind[s] = s < p->n_surf ? p->surf_kind[s] : 0;
-    P.surf_W[s] = s < p->n_surf ? p->surf_W[s] : 0.0f;
-    for (int c = 0; c < 3; ++c) P.surf_dir[s][c] = s < p->n_surf ? p->surf_dir[s][c] : 0.0f;
-  }
-  P.seed_lo = (uint32_t)(p->seed & 0xFFFFFFFFu);
-  P.seed_hi = (uint32_t)(p->seed >> 32);
-  P.stream = p->stream;
-  P.nt = nt;
-  P.valid = valid;
+  mc_to_dev_params(p, &h->P);
+  h->P.nt = nt;
+  h->P.valid = valid;
   h->order = p->order;
 
   const size_t fbytes = (size_t)nt * sizeof(float);
@@ -400,6 +411,89 @@ int mc_get_sweep_kernel(mc_handle *h, int *kind) {
   if (rc) return rc;
   if (!kind) return fail(MC_EINVAL, "mc_get_sweep_kernel: null output");
   *kind = use_x4(h) ? MC_KERNEL_X4 : MC_KERNEL_GENERIC;
+  return MC_OK;
+}
+
+int mc_set_tuning(mc_handle *h, int grid, int xcd) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (grid < 0 || (xcd && grid % 8) || (xcd != 0 && xcd != 1))
+    return fail(MC_EINVAL, "mc_set_tuning: grid must be >= 0 (0 = default) and a multiple of 8 with xcd = 1");
+  MC_HIP(hipStreamSynchronize(h->stream));
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  h->x4_grid = grid;
+  h->x4_xcd = xcd;
+  return MC_OK;
+}
+
+int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!use_x4(h)) {
+    if (grid) *grid = 0;
+    if (xcd) *xcd = 0;
+    return MC_OK;
+  }
+  // Candidates run on a scratch copy of the lattice and state, so tuning
+  // leaves the chain untouched.
+  const size_t nt = (size_t)h->P.nt;
+  float *scratch = nullptr;
+  MCDevState *sst = nullptr;
+  MC_HIP(hipMalloc(&scratch, 3 * nt * sizeof(float)));
+  if (hipMalloc(&sst, sizeof(MCDevState)) != hipSuccess) {
+    (void)hipFree(scratch);
+    return fail(MC_ENOMEM, "mc_autotune: scratch state allocation failed");
+  }
+  MCLaunch L = h->launch();
+  L.nx = scratch;
+  L.ny = scratch + nt;
+  L.nz = scratch + 2 * nt;
+  L.st = sst;
+  int best_g = 0, best_x = 0;
+  float best = 1e30f;
+  const int mults[] = {2, 4, 8, 16, 32};
+  hipError_t err = hipSuccess;
+  for (int xm = 0; xm < 2 && err == hipSuccess; ++xm) {
+    for (int m : mults) {
+      const int g = h->cu_count * m;
+      if (xm && g % 8) continue;
+      err = hipMemcpyAsync(scratch, h->nx, nt * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+      if (err == hipSuccess) err = hipMemcpyAsync(scratch + nt, h->ny, nt * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+      if (err == hipSuccess) err = hipMemcpyAsync(scratch + 2 * nt, h->nz, nt * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+      if (err == hipSuccess) err = hipMemcpyAsync(sst, h->st, sizeof(MCDevState), hipMemcpyDeviceToDevice, h->stream);
+      if (err != hipSuccess) break;
+      for (int w = 0; w < 2; ++w) {  // warm-up
+        mc_launch_sweep_x4(L, 0, g, 0, xm, nullptr);
+        mc_launch_sweep_x4(L, 1, g, 1, xm, nullptr);
+      }
+      const int reps = 6;
+      err = hipEventRecord(h->ev0, h->stream);
+      for (int w = 0; w < reps; ++w) {
+        mc_launch_sweep_x4(L, 0, g, 0, xm, nullptr);
+        mc_launch_sweep_x4(L, 1, g, 1, xm, nullptr);
+      }
+      if (err == hipSuccess) err = hipEventRecord(h->ev1, h->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(h->ev1);
+      float ms = 0.0f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+      if (err != hipSuccess) break;
+      const float us = ms * 1e3f / reps;
+      if (us < best) {
+        best = us;
+        best_g = g;
+        best_x = xm;
+      }
+    }
+  }
+  (void)hipFree(scratch);
+  (void)hipFree(sst);
+  MC_HIP(err);
+  rc = mc_set_tuning(h, best_g, best_x);
+  if (rc) return rc;
+  if (grid) *grid = best_g;
+  if (xcd) *xcd = best_x;
+  if (best_us) *best_us = best;
   return MC_OK;
 }
 

@@ -295,3 +295,28 @@ __device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, ui
   st->sweep = st->sweep + 1u;
 }
 
+
+// Tile schedule of the persistent sweep kernels.  Tiles are the 256-item
+// units one workgroup handles per iteration.  xcd == 0: plain grid stride.
+// xcd == 1: the dispatcher hands workgroups to the 8 XCDs round robin, so
+// workgroup b runs on XCD b % 8; each XCD gets one contiguous eighth of the
+// tiles and strides through it, which keeps neighbouring rows (the halos of a
+// tile) in the same XCD's L2.  Requires gridDim.x % 8 == 0 for xcd == 1.
+struct MCTileRange {
+  int first, end, step;
+};
+__device__ __forceinline__ MCTileRange mcd_tiles(int ntile, int xcd) {
+  MCTileRange r;
+  if (!xcd) {
+    r.first = blockIdx.x;
+    r.end = ntile;
+    r.step = gridDim.x;
+  } else {
+    const int x = blockIdx.x & 7, lb = blockIdx.x >> 3;
+    const int per = (ntile + 7) >> 3;
+    r.first = x * per + lb;
+    r.end = min(ntile, (x + 1) * per);
+    r.step = gridDim.x >> 3;
+  }
+  return r;
+}

@@ -13,6 +13,14 @@ struct MCRecord {
   float *eo, *en;
 };
 
+// Sets the message mc_last_error() returns on this thread.
+void mc_set_error_message(const char *msg);
+
+// MCDevParams from the public parameter block (shared by mc_create and
+// mc_batch_create); returns an error message or nullptr.
+struct mc_params;
+const char *mc_to_dev_params(const mc_params *p, MCDevParams *P);
+
 // Everything a launch needs about one lattice.
 struct MCLaunch {
   MCDevParams P;
@@ -30,7 +38,26 @@ void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *pr
 // Fast pass for all-active periodic lattices with Nx % 4 == 0 (mc_sweep_x4.hip).
 bool mc_x4_eligible(const MCLaunch &L);
 int mc_x4_slots(const MCDevParams &P);
-void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, const MCRecord *rec);
+void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec);
+// Batched replicas (mc_sweep_x4.hip / mc_batch.hip).
+struct MCBatchLaunch {
+  hipStream_t stream;
+  int potential;
+  int nrep, ntiles;
+  const MCDevParams *params;  // per replica
+  MCDevState *states;         // per replica
+  const int *tile_rep;        // replica of each tile
+  const int *rep_tile0;       // first tile of each replica
+  const int *rep_ntiles;      // tiles of each replica
+  const long long *rep_off;   // first site of each replica in the planes
+  float *nx, *ny, *nz;
+  uint32_t *blocks_done;
+  int xcd;  // tile schedule (mcd_tiles)
+};
+void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finalize);
+// Per-replica FP64 {E, Qxx..Qzz, n} partial sums per tile, then per-replica
+// reduction in tile order: out[r*8..] = {E_mean, S, q6}.
+void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
 // out = {E, S, q6, n_active}; series (may be null) receives {E, S}.
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series);

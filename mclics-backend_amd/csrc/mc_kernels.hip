@@ -346,6 +346,98 @@ __global__ void k_soa_to_aos(const float *__restrict__ x, const float *__restric
   }
 }
 
+// Batched replicas: per-tile FP64 partial sums {E, Qxx..Qzz, n} (a tile is
+// 256 segments = 1024 consecutive sites of one replica).
+template <int POT>
+__global__ __launch_bounds__(kBlock) void k_batch_measure(const MCDevParams *__restrict__ params,
+                                                          const MCDevState *__restrict__ states,
+                                                          const int *__restrict__ tile_rep,
+                                                          const int *__restrict__ rep_tile0,
+                                                          const long long *__restrict__ rep_off, int ntiles,
+                                                          const float *__restrict__ nx, const float *__restrict__ ny,
+                                                          const float *__restrict__ nz, double *__restrict__ partials) {
+  __shared__ double red[8][kBlock / 64];
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int r = tile_rep[tile];
+    const MCDevParams P = params[r];
+    const float elecE = states[r].elecE;
+    const double cA = (double)(-P.A) * 1.5;
+    const long long off = rep_off[r];
+    const float *X = nx + off, *Y = ny + off, *Z = nz + off;
+    const int t = (tile - rep_tile0[r]) * kBlock + threadIdx.x;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < (P.nx >> 2) * P.ny * P.nz) {
+      for (int e = 0; e < 4; ++e) {
+        const int s = 4 * t + e;
+        const int i = s % P.nx, j = (s / P.nx) % P.ny, k = s / (P.nx * P.ny);
+        const float o0 = X[s], o1 = Y[s], o2 = Z[s];
+        float Eo, En;
+        mcd_site_energy<POT, false, false>(P, cA, X, Y, Z, nullptr, nullptr, nullptr, nullptr, elecE, i, j, k, s,
+                                           o0, o1, o2, 0.f, 0.f, 0.f, Eo, En);
+        const double x = o0, y = o1, z = o2;
+        a[0] += (double)Eo;
+        a[1] += 3.0 * x * x - 1.0;
+        a[2] += 3.0 * x * y;
+        a[3] += 3.0 * x * z;
+        a[4] += 3.0 * y * y - 1.0;
+        a[5] += 3.0 * y * z;
+        a[6] += 3.0 * z * z - 1.0;
+        a[7] += 1.0;
+      }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double v = a[c];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      if (lane == 0) red[c][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      double v = 0;
+      for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+      partials[(size_t)tile * 8 + threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per replica sums its tiles in tile order (deterministic):
+// out[r*8..] = {E_mean, S, q6}; series[r*2..] = {E, S} when given.
+__global__ __launch_bounds__(kBlock) void k_batch_measure_final(const int *__restrict__ rep_tile0,
+                                                                const int *__restrict__ rep_ntiles,
+                                                                const double *__restrict__ partials,
+                                                                double *__restrict__ out, double *__restrict__ series) {
+  __shared__ double red[8][kBlock];
+  const int r = blockIdx.x;
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int p = threadIdx.x; p < rep_ntiles[r]; p += kBlock)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a[c] += partials[(size_t)(rep_tile0[r] + p) * 8 + c];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) red[c][threadIdx.x] = a[c];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = red[7][0];
+    double q[6];
+    for (int c = 0; c < 6; ++c) q[c] = red[c + 1][0] / (2.0 * n);
+    double *o = out + (size_t)r * 8;
+    o[0] = red[0][0] / n;
+    o[1] = mcd_sym3_max(q);
+    for (int c = 0; c < 6; ++c) o[2 + c] = q[c];
+    if (series) {
+      series[2 * (size_t)r] = o[0];
+      series[2 * (size_t)r + 1] = o[1];
+    }
+  }
+}
+
 template <int POT, bool GEN>
 void launch_sweep_pass_t(const MCLaunch &L, int set, int total, int grid, int finalize, const MCRecord *rec) {
   if (rec)
@@ -425,4 +517,22 @@ void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, f
 void mc_launch_soa_to_aos(hipStream_t s, const float *x, const float *y, const float *z, float *aos, int n,
                           int grid) {
   k_soa_to_aos<<<grid, kBlock, 0, s>>>(x, y, z, aos, n);
+}
+
+void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series) {
+  switch (B.potential) {
+    case MCD_GHRL:
+      k_batch_measure<MCD_GHRL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                               B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+    case MCD_PEAR:
+      k_batch_measure<MCD_PEAR><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                               B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+    default:
+      k_batch_measure<MCD_LL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                             B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+  }
+  k_batch_measure_final<<<B.nrep, kBlock, 0, B.stream>>>(B.rep_tile0, B.rep_ntiles, tile_partials, out, series);
 }

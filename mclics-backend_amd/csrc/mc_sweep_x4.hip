@@ -208,7 +208,7 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, float *__restr
 template <int POT, bool RECORD>
 __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__restrict__ nx, float *__restrict__ ny,
                                                      float *__restrict__ nz, MCDevState *st, int colour, int finalize,
-                                                     MCRecord rec) {
+                                                     int xcd, MCRecord rec) {
   const float T = st->T, elecE = st->elecE, av = st->ang_var;
   const uint32_t sweep = st->sweep;
   const double cA = (double)(-P.A) * 1.5;
@@ -218,7 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
   const int nseg = P.nx >> 2;
   const int total = nseg * P.ny * P.nz;
   uint32_t acc = 0;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+  const MCTileRange tr = mcd_tiles((total + kBlock - 1) / kBlock, xcd);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int t = tile * kBlock + threadIdx.x;
+    if (t >= total) continue;
     const int q = t % nseg;
     const int rowi = t / nseg;
     const int j = rowi % P.ny, k = rowi / P.ny;
@@ -232,11 +235,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
 }
 
 template <int POT>
-void launch_t(const MCLaunch &L, int colour, int grid, int finalize, const MCRecord *rec) {
+void launch_t(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
   if (rec)
-    k_sweep_x4<POT, true><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, *rec);
+    k_sweep_x4<POT, true><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd, *rec);
   else
-    k_sweep_x4<POT, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, MCRecord{});
+    k_sweep_x4<POT, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
+                                                          MCRecord{});
 }
 
 }  // namespace
@@ -247,16 +251,117 @@ bool mc_x4_eligible(const MCLaunch &L) {
 
 int mc_x4_slots(const MCDevParams &P) { return (P.nx / 4) * P.ny * P.nz; }
 
-void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, const MCRecord *rec) {
+void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
   switch (L.P.potential) {
     case MCD_GHRL:
-      launch_t<MCD_GHRL>(L, colour, grid, finalize, rec);
+      launch_t<MCD_GHRL>(L, colour, grid, finalize, xcd, rec);
       break;
     case MCD_PEAR:
-      launch_t<MCD_PEAR>(L, colour, grid, finalize, rec);
+      launch_t<MCD_PEAR>(L, colour, grid, finalize, xcd, rec);
       break;
     default:
-      launch_t<MCD_LL>(L, colour, grid, finalize, rec);
+      launch_t<MCD_LL>(L, colour, grid, finalize, xcd, rec);
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched replicas: R independent lattices (uneven sizes, each all active,
+// periodic, Nx % 4 == 0) in one set of SoA planes.  A pass is one launch over
+// the tiles of every replica (a tile = the 256 segments one workgroup updates
+// in one iteration; tiles never straddle replicas, so each workgroup reduces
+// its acceptance count into exactly one replica counter per tile).  The
+// replica of a tile comes from a tile table; each replica keeps its own
+// parameters, temperature, proposal width and generator stream, and evolves
+// exactly as the same lattice would alone in the segment kernel.
+
+namespace {
+
+template <int POT>
+__global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restrict__ params, MCDevState *states,
+                                                     const int *__restrict__ tile_rep, const int *__restrict__ rep_tile0,
+                                                     const long long *__restrict__ rep_off, int ntiles, int nrep,
+                                                     float *__restrict__ nx, float *__restrict__ ny,
+                                                     float *__restrict__ nz, int colour, int finalize, int xcd,
+                                                     uint32_t *blocks_done) {
+  const MCTileRange tr = mcd_tiles(ntiles, xcd);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int r = tile_rep[tile];
+    const MCDevParams P = params[r];
+    MCDevState *st = states + r;
+    const float T = st->T, elecE = st->elecE, av = st->ang_var;
+    const uint32_t sweep = st->sweep;
+    const double cA = (double)(-P.A) * 1.5;
+    const float cAf = (float)cA;
+    const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
+    const float eps = 6.1035156e-5f * (fabsf(cAf) + fabsf(P.elecA) * elecE * elecE * e2);
+    const long long off = rep_off[r];
+    const int nseg = P.nx >> 2;
+    const int total = nseg * P.ny * P.nz;
+    const int t = (tile - rep_tile0[r]) * kBlock + threadIdx.x;
+    uint32_t acc = 0;
+    if (t < total) {
+      const int q = t % nseg;
+      const int rowi = t / nseg;
+      const int j = rowi % P.ny, k = rowi / P.ny;
+      if (((j + k + colour) & 1) == 0)
+        acc = segment<POT, false, 0>(P, nx + off, ny + off, nz + off, q, j, k, sweep, T, elecE, av, cA, cAf, eps,
+                                     MCRecord{});
+      else
+        acc = segment<POT, false, 1>(P, nx + off, ny + off, nz + off, q, j, k, sweep, T, elecE, av, cA, cAf, eps,
+                                     MCRecord{});
+    }
+    const uint32_t bsum = mcd_block_sum(acc);
+    if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
+    __syncthreads();  // wsum of mcd_block_sum is reused by the next tile
+  }
+  if (!finalize) return;
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(blocks_done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = threadIdx.x; r < nrep; r += kBlock) {
+    MCDevState *st = states + r;
+    const uint32_t acc = atomicAdd(&st->acc_count, 0u);
+    float a = st->ang_var;
+    const double rate = 1.0 * (double)acc / (double)params[r].valid;
+    if (rate < 0.5) {
+      a = (float)((double)a * 0.99);
+      if ((double)a < 0.01) a = (float)0.01;
+    } else if (rate > 0.5) {
+      a = (float)((double)a / 0.99);
+      if ((double)a > 1.0) a = (float)((double)a - 0.5);
+    }
+    st->ang_var = a;
+    st->last_acc = acc;
+    st->acc_count = 0u;
+    st->sweep = st->sweep + 1u;
+  }
+  if (threadIdx.x == 0) *blocks_done = 0u;
+}
+
+}  // namespace
+
+void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finalize) {
+  switch (B.potential) {
+    case MCD_GHRL:
+      k_batch_x4<MCD_GHRL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
+                                                          B.xcd, B.blocks_done);
+      break;
+    case MCD_PEAR:
+      k_batch_x4<MCD_PEAR><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
+                                                          B.xcd, B.blocks_done);
+      break;
+    default:
+      k_batch_x4<MCD_LL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
+                                                        B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
+                                                        B.xcd, B.blocks_done);
       break;
   }
 }

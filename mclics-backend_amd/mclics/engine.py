@@ -57,6 +57,20 @@ EXPORTS = (
     "mc_get_acceptance",
     "mc_pass",
     "mc_record_sweep",
+    "mc_set_tuning",
+    "mc_autotune",
+    "mc_batch_create",
+    "mc_batch_destroy",
+    "mc_batch_set_T",
+    "mc_batch_sweeps",
+    "mc_batch_sync",
+    "mc_batch_time_sweeps",
+    "mc_batch_measure",
+    "mc_batch_measure_series",
+    "mc_batch_get_directors_aos",
+    "mc_batch_get_state",
+    "mc_batch_set_tuning",
+    "mc_batch_autotune",
 )
 
 
@@ -139,6 +153,23 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_get_acceptance.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.mc_pass.argtypes = [vp, C.c_int, f32p, f32p, u8p, f32p, f32p]
     lib.mc_record_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p, f32p]
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    u32p = C.POINTER(C.c_uint32)
+    lib.mc_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.mc_autotune.argtypes = [vp, ip, ip, f32p]
+    lib.mc_batch_create.argtypes = [C.c_int, C.POINTER(McParams), C.POINTER(f32p), C.POINTER(vp)]
+    lib.mc_batch_destroy.argtypes = [vp]
+    lib.mc_batch_set_T.argtypes = [vp, C.c_int, C.c_float]
+    lib.mc_batch_sweeps.argtypes = [vp, C.c_int]
+    lib.mc_batch_sync.argtypes = [vp]
+    lib.mc_batch_time_sweeps.argtypes = [vp, C.c_int, f32p]
+    lib.mc_batch_measure.argtypes = [vp, dp, dp]
+    lib.mc_batch_measure_series.argtypes = [vp, C.c_int, dp, dp]
+    lib.mc_batch_get_directors_aos.argtypes = [vp, C.c_int, f32p]
+    lib.mc_batch_get_state.argtypes = [vp, u32p, u32p, f32p]
+    lib.mc_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.mc_batch_autotune.argtypes = [vp, ip, ip, f32p]
     for name in EXPORTS:
         if name not in ("mc_last_error", "mc_device_count"):
             getattr(lib, name).restype = C.c_int
@@ -282,6 +313,14 @@ class Lattice:
         ni = np.ascontiguousarray(ni_aos, dtype=np.float32).reshape(3 * self.nt)
         _check(self._lib.mc_set_directors_aos(self._h, _f32(ni)))
 
+    def set_tuning(self, grid: int, xcd: int) -> None:
+        _check(self._lib.mc_set_tuning(self._h, grid, xcd))
+
+    def autotune(self):
+        g, x, us = C.c_int(), C.c_int(), C.c_float()
+        _check(self._lib.mc_autotune(self._h, C.byref(g), C.byref(x), C.byref(us)))
+        return g.value, x.value, us.value
+
     def set_kernel(self, kind: int) -> None:
         _check(self._lib.mc_set_sweep_kernel(self._h, kind))
 
@@ -342,6 +381,89 @@ class Lattice:
         _check(self._lib.mc_record_sweep(self._h, _f32(r["u0"]), _f32(r["u1"]), _f32(r["u2"]), _f32(r["prop"]),
                                          _u8(r["accept"]), _f32(r["e_old"]), _f32(r["e_new"])))
         return r
+
+
+class Batch:
+    """Many independent bulk lattices swept together (an mc_batch)."""
+
+    def __init__(self, cfgs: Sequence[Config], ni_list: Sequence[np.ndarray]):
+        lib = load_library()
+        self.cfgs = list(cfgs)
+        self.nrep = len(cfgs)
+        self._arr = (McParams * self.nrep)(*[c.to_struct() for c in cfgs])
+        self._ni = [np.ascontiguousarray(n, dtype=np.float32).reshape(-1) for n in ni_list]
+        ptrs = (C.POINTER(C.c_float) * self.nrep)(*[_f32(n) for n in self._ni])
+        h = C.c_void_p()
+        _check(lib.mc_batch_create(self.nrep, self._arr, ptrs, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.mc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_T(self, rep: int, T: float):
+        _check(self._lib.mc_batch_set_T(self._h, rep, T))
+
+    def sweeps(self, n: int):
+        _check(self._lib.mc_batch_sweeps(self._h, n))
+
+    def sync(self):
+        _check(self._lib.mc_batch_sync(self._h))
+
+    def time_sweeps(self, n: int) -> float:
+        ms = C.c_float()
+        _check(self._lib.mc_batch_time_sweeps(self._h, n, C.byref(ms)))
+        return ms.value
+
+    def measure(self):
+        E = np.empty(self.nrep)
+        S = np.empty(self.nrep)
+        _check(self._lib.mc_batch_measure(self._h, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                          S.ctypes.data_as(C.POINTER(C.c_double))))
+        return E, S
+
+    def measure_series(self, n: int):
+        E = np.empty((n, self.nrep))
+        S = np.empty((n, self.nrep))
+        _check(self._lib.mc_batch_measure_series(self._h, n, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 S.ctypes.data_as(C.POINTER(C.c_double))))
+        return E, S
+
+    def directors(self, rep: int) -> np.ndarray:
+        c = self.cfgs[rep]
+        out = np.empty(3 * c.nx * c.ny * c.nz, np.float32)
+        _check(self._lib.mc_batch_get_directors_aos(self._h, rep, _f32(out)))
+        return out
+
+    def state(self):
+        a = np.empty(self.nrep, np.uint32)
+        s = np.empty(self.nrep, np.uint32)
+        v = np.empty(self.nrep, np.float32)
+        _check(self._lib.mc_batch_get_state(self._h, a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            s.ctypes.data_as(C.POINTER(C.c_uint32)), _f32(v)))
+        return a, s, v
+
+    def set_tuning(self, grid: int, xcd: int):
+        _check(self._lib.mc_batch_set_tuning(self._h, grid, xcd))
+
+    def autotune(self):
+        g, x, us = C.c_int(), C.c_int(), C.c_float()
+        _check(self._lib.mc_batch_autotune(self._h, C.byref(g), C.byref(x), C.byref(us)))
+        return g.value, x.value, us.value
 
 
 def device_count() -> int:
