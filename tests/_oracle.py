@@ -197,3 +197,79 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().orc_philox4x32_10(c, k, o)
     return list(o)
+
+
+# ---------------------------------------------------------------------------
+# Learned-proposal network oracle (oracle/mlp_oracle.c).
+
+MLP_NPARAM = 64 * 22 + 64 + 64 * 64 + 64 + 4 * 64 + 4
+
+
+def _mlp_lib():
+    L = lib()
+    if not getattr(L, "_mlp_ready", False):
+        f32p, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.orc_mlp_features.argtypes = [f32p, f32p, C.c_float, f32p, f32p]
+        L.orc_bf16.argtypes = [C.c_float]
+        L.orc_bf16.restype = C.c_float
+        L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_int, dp]
+        L.orc_mlp_head.argtypes = [dp, f32p, dp, dp]
+        L.orc_mlp_nll_grad.argtypes = [f32p, C.c_int, f32p, f32p, f32p, C.c_int, dp]
+        L.orc_mlp_nll_grad.restype = C.c_double
+        L.orc_vmf_logq.argtypes = [dp, C.c_double, dp]
+        L.orc_vmf_logq.restype = C.c_double
+        L._mlp_ready = True
+    return L
+
+
+def mlp_features(nbr18, field, T):
+    L = _mlp_lib()
+    nb = np.ascontiguousarray(nbr18, np.float32)
+    fd = np.ascontiguousarray(field, np.float32)
+    x = np.zeros(22, np.float32)
+    base = np.zeros(3, np.float32)
+    L.orc_mlp_features(f32(nb), f32(fd), T, f32(x), f32(base))
+    return x, base
+
+
+def bf16(v):
+    L = _mlp_lib()
+    return np.array([L.orc_bf16(float(a)) for a in np.ravel(v)], np.float32).reshape(np.shape(v))
+
+
+def mlp_forward(params, x, use_bf16=0):
+    L = _mlp_lib()
+    out = (C.c_double * 4)()
+    p = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32)
+    L.orc_mlp_forward(f32(p), f32(xx), use_bf16, out)
+    return np.array(out[:])
+
+
+def mlp_head(out, base):
+    L = _mlp_lib()
+    o = (C.c_double * 4)(*out)
+    mu = (C.c_double * 3)()
+    lk = C.c_double()
+    b = np.ascontiguousarray(base, np.float32)
+    L.orc_mlp_head(o, f32(b), mu, C.byref(lk))
+    return np.array(mu[:]), lk.value
+
+
+def mlp_nll_grad(params, X, base, n, use_bf16=0):
+    L = _mlp_lib()
+    g = np.zeros(MLP_NPARAM, np.float64)
+    p = np.ascontiguousarray(params, np.float32)
+    XX = np.ascontiguousarray(X, np.float32)
+    bb = np.ascontiguousarray(base, np.float32)
+    nn = np.ascontiguousarray(n, np.float32)
+    loss = L.orc_mlp_nll_grad(f32(p), len(XX), f32(XX), f32(bb), f32(nn), use_bf16,
+                              g.ctypes.data_as(C.POINTER(C.c_double)))
+    return loss, g
+
+
+def vmf_logq(mu, kappa, n):
+    L = _mlp_lib()
+    m = (C.c_double * 3)(*mu)
+    v = (C.c_double * 3)(*n)
+    return L.orc_vmf_logq(m, kappa, v)
