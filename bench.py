@@ -71,13 +71,35 @@ def load_traffic():
         return None
 
 
+def copy_probe(nbytes=201326592, reps=20):
+    """Device-to-device copies of a known size, the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes nbytes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the device-copy calibration for the HBM byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU z-slab decomposition is a design, not a code path)", file=sys.stderr)
@@ -114,6 +136,8 @@ def main():
             E, S, _ = lat.measure()
             obs.append({"T": T, "E": E, "S": S})
         acc, sweeps = lat.acceptance()
+        kname = "k_sweep_x4 (segment kernel, one colour pass)" if lat.kernel() == mclics.KERNEL_X4 else \
+            "k_sweep_pass (site kernel, one colour pass)"
         log(f"bench: {args.steps} sweeps in {wall:.3f} s, mean pass {pass_ms_sum / passes * 1e3:.2f} us")
     value = nt * args.steps / wall
     avg_pass_s = pass_ms_sum / passes / 1e3
@@ -137,7 +161,7 @@ def main():
                    "sweeps_per_T": per_t, "order": "checkerboard (2 passes/sweep)", "seed": 2},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
-                     "kernel": "k_sweep_pass (one colour pass)", "bytes_per_launch": bytes_per_launch,
+                     "kernel": kname, "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_pass_s * 1e6},
         "observables": obs,
         "last_acceptance": acc / nt,
