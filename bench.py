@@ -61,6 +61,65 @@ def cpu_baseline(n0, budget_s=15.0):
             "value_1core": one}
 
 
+def learned_leg(n_meas=30000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14)):
+    """Config C5: 128^3 LL near the transition.  The network is trained on
+    n_train neighbourhoods drawn from plain-sampler lattices at the three
+    temperatures, then plain and learned chains start from the same
+    equilibrated state; tau_int(S) by automatic windowing gives decorrelated
+    sweeps/s = sweeps/s / (2 tau_int)."""
+    import mclics
+    from mclics.decorrelation import tau_error, tau_int
+    nt = L5**3
+    out = {"workload": f"LL bulk {L5}^3 periodic, A=1, T in {list(temps)}", "n_train": n_train,
+           "measured_sweeps": n_meas, "per_T": []}
+    starts = {}
+    with mclics.Network(seed=5) as net:
+        per_T = n_train // len(temps)
+        t0 = time.perf_counter()
+        for ti, T in enumerate(temps):
+            lat = mclics.Lattice(mclics.Config(L5, L5, L5, T=T, seed=50 + ti), mclics.random_ic(L5, L5, L5, seed=50 + ti))
+            lat.sweeps(3000, graph=True)
+            snaps = 4
+            for k in range(snaps):
+                lat.sweeps(50, graph=True)
+                net.add_samples(lat, per_T // snaps + (n_train - per_T * len(temps) if ti == 0 and k == 0 else 0),
+                                seed=1000 * ti + k)
+            starts[T] = lat.directors()
+            lat.close()
+        losses = net.train(epochs=10, batch=2048, lr=2e-3)
+        out["train_seconds"] = time.perf_counter() - t0
+        out["loss_per_epoch"] = [float(v) for v in losses]
+        gains = []
+        for ti, T in enumerate(temps):
+            row = {"T": T}
+            for method in ("plain", "learned"):
+                with mclics.Lattice(mclics.Config(L5, L5, L5, T=T, seed=90 + ti), starts[T]) as lat:
+                    if method == "learned":
+                        lat.set_learned(net)
+                        lat.sweeps(200)
+                        ms, _ = lat.time_sweeps(300, graph=False)
+                        sps = 300 / (ms / 1e3)
+                    else:
+                        lat.sweeps(200, graph=True)
+                        ms, _ = lat.time_sweeps(1000, graph=True)
+                        sps = 1000 / (ms / 1e3)
+                    E, S = lat.measure_series(n_meas)
+                    acc, _ = lat.acceptance()
+                tau, M = tau_int(S)
+                row[method] = {"sweeps_per_s": sps, "tau_int_S": tau, "tau_err": tau_error(tau, M, len(S)),
+                               "window": M, "decorrelated_sweeps_per_s": sps / (2 * tau), "E": float(E.mean()),
+                               "S": float(S.mean()), "acceptance": acc / nt}
+                log(f"bench: learned leg T={T} {method}: {sps:.1f} sweeps/s tau={tau:.1f} acc={acc / nt:.3f}")
+            row["gain"] = row["learned"]["decorrelated_sweeps_per_s"] / row["plain"]["decorrelated_sweeps_per_s"]
+            flops = 11520.0 * nt * row["learned"]["sweeps_per_s"]
+            row["learned"]["mlp_tflops"] = flops / 1e12
+            row["learned"]["mfma_util_bf16_dense"] = flops / 2.5e15
+            gains.append(row["gain"])
+            out["per_T"].append(row)
+        out["gain_geomean"] = float(np.exp(np.mean(np.log(gains))))
+    return out
+
+
 def load_traffic():
     path = os.path.join(REPO, "profiles", "sweep_pass_ll256_traffic.json")
     if not os.path.exists(path):
@@ -94,6 +153,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--learned-sweeps", type=int, default=30000,
+                    help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the device-copy calibration for the HBM byte counters")
     args = ap.parse_args()
@@ -166,6 +228,9 @@ def main():
         "observables": obs,
         "last_acceptance": acc / nt,
     }
+    if not args.no_learned:
+        log("bench: learned-proposal leg (128^3, T near T_NI)")
+        result["learned_proposal"] = learned_leg(args.learned_sweeps)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         result["cpu_baseline"] = cpu_baseline(n0)

@@ -190,6 +190,42 @@ int mc_batch_get_state(mc_batch *b, uint32_t *accepted, uint32_t *sweeps, float 
 int mc_batch_set_tuning(mc_batch *b, int grid, int xcd);
 int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us);
 
+/* ------------------------------------------------------------------------
+ * Learned proposal.  A 22-64-64-4 network (bf16 weights on the device, FP32
+ * master copy) maps a site's six neighbour directors (sign-canonicalised), the
+ * field vector elecE*e and 1/T to an axial von Mises-Fisher proposal
+ *   q(n) = kappa cosh(kappa mu.n) / (4 pi sinh kappa),
+ * mu = normalize(sum of canonical neighbours + o[0..2]),
+ * log kappa = clamp(o[3], -6, 8).  Sweeps that use it accept a move o -> n with
+ * probability min(1, exp(-dE/T) cosh(kappa mu.o) / cosh(kappa mu.n)), exact
+ * Metropolis-Hastings because mu and kappa depend only on the neighbours.
+ * Parameters are packed W1[64][22] b1[64] W2[64][64] b2[64] W3[4][64] b3[4]
+ * (5892 floats).  No reference counterpart. */
+#define MC_MLP_PARAMS 5892
+typedef struct mc_mlp mc_mlp;
+/* params NULL: deterministic initialisation from seed. */
+int mc_mlp_create(const float *params, uint64_t seed, mc_mlp **out);
+int mc_mlp_destroy(mc_mlp *m);
+int mc_mlp_get_params(mc_mlp *m, float *params);
+int mc_mlp_set_params(mc_mlp *m, const float *params);
+/* Inference on host arrays: X[n][22] features, base[n][3] -> mu[n][3], logk[n]. */
+int mc_mlp_infer(mc_mlp *m, int n, const float *X, const float *base, float *mu, float *logk);
+/* Mean negative log-likelihood of dirs[n][3] and its gradient (float[5892]). */
+int mc_mlp_grad(mc_mlp *m, int n, const float *X, const float *base, const float *dirs, double *loss, float *grad);
+/* Appends n training neighbourhoods drawn at random sites of the lattice's
+ * current state (its current T and field). */
+int mc_mlp_dataset_add(mc_mlp *m, mc_handle *h, int n, uint32_t seed);
+int mc_mlp_dataset_size(mc_mlp *m, int *n);
+int mc_mlp_dataset_clear(mc_mlp *m);
+/* Adam (beta 0.9/0.999) over the training set; mean loss per epoch. */
+int mc_mlp_train(mc_mlp *m, int epochs, int batch, float lr, float *loss_per_epoch);
+/* Attach (every > 0: every k-th sweep uses the network) or detach (NULL or 0). */
+int mc_set_learned_proposal(mc_handle *h, mc_mlp *net, int every);
+/* One learned sweep recording per site mu (AoS), log kappa, the proposal
+ * (AoS), the accept uniform, the decision and the FP32 energy change. */
+int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_aos, float *u, uint8_t *accept,
+                            float *dE);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@
 
 #include "../../include/mclics_hip.h"
 #include "mc_kernels.h"
+#include "mc_mlp.h"
 
 namespace {
 
@@ -62,6 +63,8 @@ __global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var
 
 void mc_set_error_message(const char *msg) { g_err = msg; }
 
+int mc_handle_launch(mc_handle *h, MCLaunch *out);
+
 struct mc_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -70,6 +73,9 @@ struct mc_handle {
   int kernel = MC_KERNEL_AUTO;  // sweep kernel selection
   int x4_grid = 0;              // workgroups of a segment pass (0: 8 per CU)
   int x4_xcd = 0;               // tile schedule of the segment pass
+  mc_mlp *learned = nullptr;    // learned proposal network (null: plain sweeps)
+  int learned_every = 0;        // every k-th sweep uses the learned proposal
+  long long host_sweeps = 0;    // sweeps enqueued so far (selects learned sweeps)
   int cu_count = 256;
   float *nx = nullptr, *ny = nullptr, *nz = nullptr;
   uint8_t *pt = nullptr;
@@ -139,12 +145,21 @@ bool use_x4(const mc_handle *h) {
 
 int passes_per_sweep(const mc_handle *h) { return h->order == MC_ORDER_REF8 ? 8 : 2; }
 
+bool learned_sweep(const mc_handle *h) {
+  return h->learned && h->learned_every > 0 && (h->host_sweeps % h->learned_every) == 0;
+}
+
 // Pass t (0-based) of one sweep; the last pass adapts ang_var.
-void launch_pass(mc_handle *h, int t, const MCRecord *rec) {
+void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *lrec = nullptr) {
   const MCLaunch L = h->launch();
   const int per = passes_per_sweep(h);
   const int finalize = t == per - 1;
-  if (use_x4(h)) {
+  if (learned_sweep(h)) {
+    const int slots = mcd_set_slots(h->P, 8 + t);
+    int grid = (slots + 63) / 64;
+    if (grid > h->cu_count * 16) grid = h->cu_count * 16;
+    mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
+  } else if (use_x4(h)) {
     const int g = h->x4_grid > 0 ? h->x4_grid : h->cu_count * 8;
     mc_launch_sweep_x4(L, t, g, finalize, h->x4_xcd, rec);
   } else {
@@ -153,8 +168,9 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec) {
   }
 }
 
-int enqueue_sweep(mc_handle *h, const MCRecord *rec) {
-  for (int t = 0; t < passes_per_sweep(h); ++t) launch_pass(h, t, rec);
+int enqueue_sweep(mc_handle *h, const MCRecord *rec, const MCLearnRecord *lrec = nullptr) {
+  for (int t = 0; t < passes_per_sweep(h); ++t) launch_pass(h, t, rec, lrec);
+  ++h->host_sweeps;
   MC_LAUNCHED("sweep pass");
   return MC_OK;
 }
@@ -513,6 +529,7 @@ int mc_sweeps_graph(mc_handle *h, int n) {
   if (rc) return rc;
   if (n < 0) return fail(MC_EINVAL, "mc_sweeps_graph: negative sweep count");
   if (n == 0) return MC_OK;
+  if (h->learned) return mc_sweeps(h, n);  // the learned/plain pattern depends on the sweep count
   auto it = h->graphs.find(n);
   if (it == h->graphs.end()) {
     hipGraph_t g = nullptr;
@@ -547,6 +564,7 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   int rc = check_handle(h);
   if (rc) return rc;
   if (n < 1 || !ms) return fail(MC_EINVAL, "mc_time_sweeps: need n >= 1 and an output");
+  if (use_graph && h->learned) use_graph = 0;
   if (use_graph) {
     // Instantiate before timing so capture cost stays outside the region.
     if (h->graphs.find(n) == h->graphs.end()) {
@@ -762,4 +780,65 @@ int mc_record_sweep(mc_handle *h, float *u0, float *u1, float *u2, float *prop_a
   return MC_OK;
 }
 
+int mc_set_learned_proposal(mc_handle *h, mc_mlp *net, int every) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (every < 0) return fail(MC_EINVAL, "mc_set_learned_proposal: every must be >= 0");
+  if (net && every > 0) {
+    if (h->order != MC_ORDER_CHECKER || h->pt)
+      return fail(MC_EINVAL, "mc_set_learned_proposal: the learned proposal needs the checkerboard order and an "
+                             "all-active lattice");
+    if (mc_mlp_device(net) != h->device)
+      return fail(MC_EINVAL, "mc_set_learned_proposal: network and lattice live on different devices");
+  }
+  MC_HIP(hipStreamSynchronize(h->stream));
+  h->learned = every > 0 ? net : nullptr;
+  h->learned_every = net ? every : 0;
+  h->host_sweeps = 0;
+  return MC_OK;
+}
+
+int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_aos, float *u, uint8_t *accept,
+                            float *dE) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!h->learned) return fail(MC_ESTATE, "mc_record_learned_sweep: no learned proposal attached");
+  rc = ensure_parity_buffers(h);
+  if (rc) return rc;
+  const size_t nt = (size_t)h->P.nt;
+  MCLearnRecord rec;
+  rec.mu = h->staging;  // 3 planes, AoS
+  rec.prop = h->pbuf;   // 3 planes, AoS
+  rec.logk = h->pbuf + 3 * nt;
+  rec.u = h->pbuf + 4 * nt;
+  rec.dE = h->pbuf + 5 * nt;
+  rec.acc = h->abuf;
+  MC_HIP(hipMemsetAsync(h->pbuf, 0, 6 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->staging, 0, 3 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->abuf, 0, nt, h->stream));
+  const long long keep = h->host_sweeps;
+  h->host_sweeps = 0;  // force a learned sweep
+  const int every = h->learned_every;
+  h->learned_every = 1;
+  rc = enqueue_sweep(h, nullptr, &rec);
+  h->learned_every = every;
+  h->host_sweeps = keep + 1;
+  if (rc) return rc;
+  if (mu) MC_HIP(hipMemcpyAsync(mu, rec.mu, 3 * nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (prop_aos) MC_HIP(hipMemcpyAsync(prop_aos, rec.prop, 3 * nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (logk) MC_HIP(hipMemcpyAsync(logk, rec.logk, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (u) MC_HIP(hipMemcpyAsync(u, rec.u, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (dE) MC_HIP(hipMemcpyAsync(dE, rec.dE, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (accept) MC_HIP(hipMemcpyAsync(accept, rec.acc, nt, hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
 }  // extern "C"
+
+int mc_handle_launch(mc_handle *h, MCLaunch *out) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  *out = h->launch();
+  return MC_OK;
+}

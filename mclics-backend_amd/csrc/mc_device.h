@@ -88,6 +88,36 @@ __device__ __forceinline__ MCU4 mcd_philox(MCU4 c, uint32_t k0, uint32_t k1) {
 __device__ __forceinline__ float mcd_u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
 // ---------------------------------------------------------------------------
+// Site sets of a pass.
+
+// Maps a linear slot of a site set to (i, j, k); returns false for slots
+// outside the lattice.
+__device__ __forceinline__ bool mcd_set_site(const MCDevParams &P, int set, int t, int &i, int &j, int &k) {
+  if (set >= 8) {
+    const int c = set - 8;
+    const int hx = (P.nx + 1) >> 1;
+    const int h = t % hx;
+    const int row = t / hx;
+    j = row % P.ny;
+    k = row / P.ny;
+    i = 2 * h + ((j + k + c) & 1);
+    return i < P.nx && k < P.nz;
+  }
+  const int di = set & 1, dj = (set >> 1) & 1, dk = set >> 2;
+  const int Ni = (P.nx + 1) >> 1, Nj = (P.ny + 1) >> 1;
+  i = di + 2 * (t % Ni);
+  j = dj + 2 * ((t / Ni) % Nj);
+  k = dk + 2 * (t / (Ni * Nj));
+  return i < P.nx && j < P.ny && k < P.nz;
+}
+
+// Number of slots of a site set (threads a pass maps).
+__device__ __host__ __forceinline__ int mcd_set_slots(const MCDevParams &P, int set) {
+  if (set >= 8) return ((P.nx + 1) / 2) * P.ny * P.nz;
+  return ((P.nx + 1) / 2) * ((P.ny + 1) / 2) * ((P.nz + 1) / 2);
+}
+
+// ---------------------------------------------------------------------------
 // Energy terms.
 
 __device__ __forceinline__ float mcd_dot(float a0, float a1, float a2, float b0, float b1, float b2) {

@@ -31,6 +31,10 @@ struct MCLaunch {
   MCDevState *st;
 };
 
+// Launch description of a handle's lattice (for the learned-proposal code).
+struct mc_handle;
+int mc_handle_launch(mc_handle *h, MCLaunch *out);
+
 int mc_set_slots(const MCDevParams &P, int set);
 void mc_launch_sweep_pass(const MCLaunch &L, int set, int grid, int finalize, const MCRecord *rec);
 void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *prop, const float *u, uint8_t *acc,

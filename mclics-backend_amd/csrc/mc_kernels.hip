@@ -18,27 +18,6 @@ namespace {
 
 constexpr int kBlock = MCD_BLOCK;
 
-// Maps a linear slot of a site set to (i, j, k); returns false for slots
-// outside the lattice.
-__device__ __forceinline__ bool mcd_set_site(const MCDevParams &P, int set, int t, int &i, int &j, int &k) {
-  if (set >= 8) {
-    const int c = set - 8;
-    const int hx = (P.nx + 1) >> 1;
-    const int h = t % hx;
-    const int row = t / hx;
-    j = row % P.ny;
-    k = row / P.ny;
-    i = 2 * h + ((j + k + c) & 1);
-    return i < P.nx && k < P.nz;
-  }
-  const int di = set & 1, dj = (set >> 1) & 1, dk = set >> 2;
-  const int Ni = (P.nx + 1) >> 1, Nj = (P.ny + 1) >> 1;
-  i = di + 2 * (t % Ni);
-  j = dj + 2 * ((t / Ni) % Nj);
-  k = dk + 2 * (t / (Ni * Nj));
-  return i < P.nx && j < P.ny && k < P.nz;
-}
-
 // Neighbour indices in the reference order +x,-x,+y,-y,+z,-z and whether each
 // bond exists: periodic axes wrap, free axes drop out-of-range neighbours
 // (src/potential.cpp:237-247), inactive neighbours drop out (pt==0).
@@ -451,10 +430,7 @@ void launch_sweep_pass_t(const MCLaunch &L, int set, int total, int grid, int fi
 }  // namespace
 
 // Number of slots of a site set (threads the pass maps).
-int mc_set_slots(const MCDevParams &P, int set) {
-  if (set >= 8) return ((P.nx + 1) / 2) * P.ny * P.nz;
-  return ((P.nx + 1) / 2) * ((P.ny + 1) / 2) * ((P.nz + 1) / 2);
-}
+int mc_set_slots(const MCDevParams &P, int set) { return mcd_set_slots(P, set); }
 
 void mc_launch_sweep_pass(const MCLaunch &L, int set, int grid, int finalize, const MCRecord *rec) {
   const int total = mc_set_slots(L.P, set);

@@ -1,0 +1,725 @@
+// Learned proposal: a 22-64-64-4 MLP that maps a site's neighbourhood to an
+// axial von Mises-Fisher proposal, trained and evaluated with bf16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) on gfx950, and used inside an
+// exact Metropolis-Hastings sweep.  No reference counterpart.
+//
+// MFMA operand layout (16x16x32): lane l of the wave holds, for A (16 x 32,
+// row-major M x K), row l%16 and the 8 K values 8*(l/16)..+7; for B (32 x 16,
+// K x N), column l%16 and the same 8 K values; the 16x16 FP32 result D has
+// D[4*(l/16)+i][l%16] in component i.  A and B always take the same K
+// assignment, so a product is correct for any internal K order; only the
+// documented M/N placement of D is relied on.
+//
+// Network and proposal, as in the CPU oracle:
+//   x = features(neighbours, field, T) (22), h1 = relu(W1 x + b1),
+//   h2 = relu(W2 h1 + b2), o = W3 h2 + b3,
+//   mu = normalize(base + o[0..2]), log kappa = clamp(o[3], -6, 8),
+//   q(n) = kappa cosh(kappa mu.n) / (4 pi sinh kappa).
+// Because mu and kappa depend on the neighbours only, which a pass never
+// changes at the updated site, the Metropolis-Hastings ratio of a move
+// o -> n is exp(-dE/T) cosh(kappa mu.o) / cosh(kappa mu.n).
+#include "mc_kernels.h"
+#include "mc_mlp.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int W1_OFF = 0, B1_OFF = 64 * 22, W2_OFF = B1_OFF + 64, B2_OFF = W2_OFF + 64 * 64,
+              W3_OFF = B2_OFF + 64, B3_OFF = W3_OFF + 4 * 64;
+
+__device__ __forceinline__ f4 mfma(bf16x8 a, bf16x8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 load8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// B operand from a row-major [rows][ld] bf16 matrix: column n0 + l%16 of the
+// product is row n0 + l%16 of W, K values k0 + 8*(l/16)..+7.
+__device__ __forceinline__ bf16x8 bop(const __bf16 *W, int ld, int n0, int k0) {
+  const int l = threadIdx.x & 63;
+  return load8(W + (size_t)(n0 + (l & 15)) * ld + k0 + 8 * (l >> 4));
+}
+
+// Stores a D tile (+bias, optional ReLU) into an LDS row-major bf16 matrix.
+__device__ __forceinline__ void store_tile(__bf16 *H, int ld, int m0, int n0, f4 c, const float *bias, bool relu) {
+  const int l = threadIdx.x & 63;
+  const int col = n0 + (l & 15);
+  const float b = bias ? bias[col] : 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float v = c[i] + b;
+    if (relu) v = v > 0.0f ? v : 0.0f;
+    H[(m0 + 4 * (l >> 4) + i) * ld + col] = (__bf16)v;
+  }
+}
+
+// Network weights in the operand-friendly bf16 layouts.
+struct MLPDev {
+  const __bf16 *W1b;   // [64][32]   (inputs 22..31 zero)
+  const __bf16 *W2b;   // [64][64]
+  const __bf16 *W3b;   // [16][64]   (rows 4..15 zero)
+  const __bf16 *W3Tb;  // [64][32]   W3 transposed (cols 4..31 zero)
+  const __bf16 *W2Tb;  // [64][64]   W2 transposed
+  const float *b1, *b2, *b3;
+};
+
+// Forward of one 64-site block: Xs (64x32 bf16) -> Os (64x16 FP32, columns
+// 0..3 meaningful).  H1s/H2s are 64x64 bf16 LDS scratch.  One wave.
+__device__ void forward64(const MLPDev &N, const bf16x8 w1[4], const bf16x8 w2[4][2], const bf16x8 w3[2],
+                          const __bf16 *Xs, __bf16 *H1s, __bf16 *H2s, float *Os) {
+  const int l = threadIdx.x & 63;
+  const int r = l & 15, g = l >> 4;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8 a = load8(Xs + (16 * mt + r) * 32 + 8 * g);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) store_tile(H1s, 64, 16 * mt, 16 * nt, mfma(a, w1[nt], f4{0, 0, 0, 0}), N.b1, true);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8 a0 = load8(H1s + (16 * mt + r) * 64 + 8 * g);
+    const bf16x8 a1 = load8(H1s + (16 * mt + r) * 64 + 32 + 8 * g);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      f4 c = mfma(a0, w2[nt][0], f4{0, 0, 0, 0});
+      c = mfma(a1, w2[nt][1], c);
+      store_tile(H2s, 64, 16 * mt, 16 * nt, c, N.b2, true);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8 a0 = load8(H2s + (16 * mt + r) * 64 + 8 * g);
+    const bf16x8 a1 = load8(H2s + (16 * mt + r) * 64 + 32 + 8 * g);
+    f4 c = mfma(a0, w3[0], f4{0, 0, 0, 0});
+    c = mfma(a1, w3[1], c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * mt + 4 * g + i;
+      Os[row * 16 + r] = c[i] + (r < 4 ? N.b3[r] : 0.0f);
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void load_weights(const MLPDev &N, bf16x8 w1[4], bf16x8 w2[4][2], bf16x8 w3[2]) {
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    w1[nt] = bop(N.W1b, 32, 16 * nt, 0);
+    w2[nt][0] = bop(N.W2b, 64, 16 * nt, 0);
+    w2[nt][1] = bop(N.W2b, 64, 16 * nt, 32);
+  }
+  w3[0] = bop(N.W3b, 64, 0, 0);
+  w3[1] = bop(N.W3b, 64, 0, 32);
+}
+
+// Features of one site, FP32 operation for operation as orc_mlp_features.
+__device__ __forceinline__ void features(const float m[6][3], float f0, float f1, float f2, float T, float x[22],
+                                         float base[3]) {
+  float c0 = m[0][0], c1 = m[0][1], c2 = m[0][2];
+  const bool flip0 = c2 < 0.0f || (c2 == 0.0f && (c1 < 0.0f || (c1 == 0.0f && c0 < 0.0f)));
+  if (flip0) {
+    c0 = -c0;
+    c1 = -c1;
+    c2 = -c2;
+  }
+  x[0] = c0;
+  x[1] = c1;
+  x[2] = c2;
+  base[0] = c0;
+  base[1] = c1;
+  base[2] = c2;
+#pragma unroll
+  for (int b = 1; b < 6; ++b) {
+    float d = c0 * m[b][0];
+    d = d + c1 * m[b][1];
+    d = d + c2 * m[b][2];
+    const float s = d < 0.0f ? -1.0f : 1.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[3 * b + c] = s * m[b][c];
+      base[c] = base[c] + s * m[b][c];
+    }
+  }
+  x[18] = f0;
+  x[19] = f1;
+  x[20] = f2;
+  x[21] = 1.0f / T;
+}
+
+__device__ __forceinline__ void write_xrow(__bf16 *Xs, int row, const float x[22], bool valid) {
+#pragma unroll
+  for (int k = 0; k < 32; ++k) Xs[row * 32 + k] = (__bf16)((valid && k < 22) ? x[k] : 0.0f);
+}
+
+// log cosh(z) + log 2, FP32.
+__device__ __forceinline__ float lcosh2(float z) {
+  const float a = fabsf(z);
+  return a + log1pf(__expf(-2.0f * a));
+}
+
+__device__ __forceinline__ void head(const float *o, const float base[3], float mu[3], float &logk) {
+  float v0 = base[0] + o[0], v1 = base[1] + o[1], v2 = base[2] + o[2];
+  float r2 = v0 * v0 + v1 * v1 + v2 * v2;
+  const float inv = r2 > 0.0f ? rsqrtf(r2) : 1.0f;
+  mu[0] = v0 * inv;
+  mu[1] = v1 * inv;
+  mu[2] = v2 * inv;
+  logk = fminf(fmaxf(o[3], -6.0f), 8.0f);
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_mlp_pack(const float *__restrict__ P, __bf16 *W1b, __bf16 *W2b, __bf16 *W3b, __bf16 *W3Tb,
+                           __bf16 *W2Tb, float *b1, float *b2, float *b3) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 64 * 64; t += gridDim.x * blockDim.x) {
+    const int j = t / 64, k = t % 64;
+    W2b[t] = (__bf16)P[W2_OFF + t];
+    W2Tb[k * 64 + j] = (__bf16)P[W2_OFF + t];
+    if (k < 32) W1b[j * 32 + k] = (__bf16)(k < 22 ? P[W1_OFF + 22 * j + k] : 0.0f);
+    if (j < 16) W3b[j * 64 + k] = (__bf16)(j < 4 ? P[W3_OFF + 64 * j + k] : 0.0f);
+    if (j < 32) W3Tb[k * 32 + j] = (__bf16)(j < 4 ? P[W3_OFF + 64 * j + k] : 0.0f);
+    if (t < 64) {
+      b1[t] = P[B1_OFF + t];
+      b2[t] = P[B2_OFF + t];
+    }
+    if (t < 4) b3[t] = P[B3_OFF + t];
+  }
+}
+
+// Inference over n neighbourhoods given as features X[n][22] and base[n][3].
+__global__ __launch_bounds__(64) void k_mlp_infer(MLPDev N, const float *__restrict__ X,
+                                                  const float *__restrict__ base, int n, float *__restrict__ mu,
+                                                  float *__restrict__ logk) {
+  __shared__ __align__(16) __bf16 Xs[64 * 32];
+  __shared__ __align__(16) __bf16 H1s[64 * 64];
+  __shared__ __align__(16) __bf16 H2s[64 * 64];
+  __shared__ float Os[64 * 16];
+  bf16x8 w1[4], w2[4][2], w3[2];
+  load_weights(N, w1, w2, w3);
+  const int l = threadIdx.x;
+  for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
+    const int s = blk * 64 + l;
+    const bool valid = s < n;
+    float x[22];
+#pragma unroll
+    for (int k = 0; k < 22; ++k) x[k] = valid ? X[(size_t)s * 22 + k] : 0.0f;
+    write_xrow(Xs, l, x, valid);
+    __syncthreads();
+    forward64(N, w1, w2, w3, Xs, H1s, H2s, Os);
+    if (valid) {
+      const float b[3] = {base[3 * (size_t)s], base[3 * (size_t)s + 1], base[3 * (size_t)s + 2]};
+      float m[3], lk;
+      head(Os + l * 16, b, m, lk);
+      mu[3 * (size_t)s] = m[0];
+      mu[3 * (size_t)s + 1] = m[1];
+      mu[3 * (size_t)s + 2] = m[2];
+      logk[s] = lk;
+    }
+    __syncthreads();
+  }
+}
+
+// Gradient of the mean negative log-likelihood over samples [0, n) (each
+// wave takes 32-sample batches): forward, loss head, backward with bf16 MFMA
+// products and FP32 accumulation; per-wave gradient tiles are added to
+// grad[5892] (FP32 atomics) and the loss sum to loss[0].
+__global__ __launch_bounds__(64) void k_mlp_grad(MLPDev N, const float *__restrict__ X,
+                                                 const float *__restrict__ base, const float *__restrict__ dirs, int n,
+                                                 float *__restrict__ grad, double *__restrict__ loss) {
+  __shared__ __align__(16) __bf16 Xs[32 * 32];
+  __shared__ __align__(16) __bf16 H1s[32 * 64];
+  __shared__ __align__(16) __bf16 H2s[32 * 64];
+  __shared__ __align__(16) __bf16 dOs[32 * 32];
+  __shared__ __align__(16) __bf16 dH2s[32 * 64];
+  __shared__ __align__(16) __bf16 dH1s[32 * 64];
+  __shared__ float Os[32 * 16];
+  const int l = threadIdx.x;
+  const int r = l & 15, g = l >> 4;
+  bf16x8 w1[4], w2[4][2], w3[2], w3t[4], w2t[4][2];
+  load_weights(N, w1, w2, w3);
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    w3t[nt] = bop(N.W3Tb, 32, 16 * nt, 0);
+    w2t[nt][0] = bop(N.W2Tb, 64, 16 * nt, 0);
+    w2t[nt][1] = bop(N.W2Tb, 64, 16 * nt, 32);
+  }
+  f4 gW1[4][2], gW2[4][4], gW3[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    gW3[a] = f4{0, 0, 0, 0};
+    for (int b = 0; b < 2; ++b) gW1[a][b] = f4{0, 0, 0, 0};
+    for (int b = 0; b < 4; ++b) gW2[a][b] = f4{0, 0, 0, 0};
+  }
+  float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.0f;
+  double lsum = 0.0;
+  for (int blk = blockIdx.x; blk * 32 < n; blk += gridDim.x) {
+    // inputs: lanes 0..31 one sample each
+    if (l < 32) {
+      const int s = blk * 32 + l;
+      float x[22];
+#pragma unroll
+      for (int k = 0; k < 22; ++k) x[k] = s < n ? X[(size_t)s * 22 + k] : 0.0f;
+      write_xrow(Xs, l, x, s < n);
+    }
+    __syncthreads();
+    // forward (2 M-tiles of 16)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const bf16x8 a = load8(Xs + (16 * mt + r) * 32 + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) store_tile(H1s, 64, 16 * mt, 16 * nt, mfma(a, w1[nt], f4{0, 0, 0, 0}), N.b1, true);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const bf16x8 a0 = load8(H1s + (16 * mt + r) * 64 + 8 * g);
+      const bf16x8 a1 = load8(H1s + (16 * mt + r) * 64 + 32 + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        f4 c = mfma(a0, w2[nt][0], f4{0, 0, 0, 0});
+        c = mfma(a1, w2[nt][1], c);
+        store_tile(H2s, 64, 16 * mt, 16 * nt, c, N.b2, true);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const bf16x8 a0 = load8(H2s + (16 * mt + r) * 64 + 8 * g);
+      const bf16x8 a1 = load8(H2s + (16 * mt + r) * 64 + 32 + 8 * g);
+      f4 c = mfma(a0, w3[0], f4{0, 0, 0, 0});
+      c = mfma(a1, w3[1], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Os[(16 * mt + 4 * g + i) * 16 + r] = c[i] + (r < 4 ? N.b3[r] : 0.0f);
+    }
+    __syncthreads();
+    // loss head: lanes 0..31, dOut into dOs (bf16, columns 0..3)
+    if (l < 32) {
+      const int s = blk * 32 + l;
+      float d[4] = {0, 0, 0, 0};
+      if (s < n) {
+        const float *o = Os + l * 16;
+        const float b0 = base[3 * (size_t)s], b1v = base[3 * (size_t)s + 1], b2v = base[3 * (size_t)s + 2];
+        const float v0 = b0 + o[0], v1 = b1v + o[1], v2 = b2v + o[2];
+        float rr = sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+        if (rr == 0.0f) rr = 1.0f;
+        const float m0 = v0 / rr, m1 = v1 / rr, m2 = v2 / rr;
+        float lk = o[3];
+        const bool clamped = lk < -6.0f || lk > 8.0f;
+        lk = fminf(fmaxf(lk, -6.0f), 8.0f);
+        const float k = __expf(lk);
+        const float n0 = dirs[3 * (size_t)s], n1 = dirs[3 * (size_t)s + 1], n2 = dirs[3 * (size_t)s + 2];
+        const float c = m0 * n0 + m1 * n1 + m2 * n2;
+        const float z = k * c;
+        // NLL = -log k + log 4pi + log sinh k - log cosh z
+        const float lsinh = k + log1pf(-__expf(-2.0f * k)) - 0.69314718f;
+        lsum += (double)(-lk + 2.5310242f + lsinh - (lcosh2(z) - 0.69314718f));
+        const float th = tanhf(z);
+        d[3] = clamped ? 0.0f : (-1.0f + k / tanhf(k) - z * th);
+        const float dm0 = -k * th * n0, dm1 = -k * th * n1, dm2 = -k * th * n2;
+        const float pr = dm0 * m0 + dm1 * m1 + dm2 * m2;
+        d[0] = (dm0 - pr * m0) / rr;
+        d[1] = (dm1 - pr * m1) / rr;
+        d[2] = (dm2 - pr * m2) / rr;
+      }
+#pragma unroll
+      for (int c = 0; c < 32; ++c) dOs[l * 32 + c] = (__bf16)(c < 4 ? d[c] : 0.0f);
+      // bias of the output layer: column sums of dOut (lane-private, reduced below)
+      Os[l * 16 + 4] = d[0];
+      Os[l * 16 + 5] = d[1];
+      Os[l * 16 + 6] = d[2];
+      Os[l * 16 + 7] = d[3];
+    }
+    __syncthreads();
+    if (l < 4)
+      for (int s = 0; s < 32; ++s) gb3 += Os[s * 16 + 4 + l];
+    // dW3[o][k] += sum_s dOut[s][o] H2[s][k]: A = dOut^T (o x s), B = H2 (s x k)
+    {
+      bf16x8 a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = dOs[(8 * g + e) * 32 + r];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = H2s[(8 * g + e) * 64 + 16 * nt + r];
+        gW3[nt] = mfma(a, b, gW3[nt]);
+      }
+    }
+    // dH2 = (dOut W3) * relu'(h2)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const bf16x8 a = load8(dOs + (16 * mt + r) * 32 + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const f4 c = mfma(a, w3t[nt], f4{0, 0, 0, 0});
+        const int col = 16 * nt + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * mt + 4 * g + i;
+          const float v = (float)H2s[row * 64 + col] > 0.0f ? c[i] : 0.0f;
+          gb2[nt] += v;
+          dH2s[row * 64 + col] = (__bf16)v;
+        }
+      }
+    }
+    __syncthreads();
+    // dW2[j][k] += sum_s dH2[s][j] H1[s][k]
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      bf16x8 a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = dH2s[(8 * g + e) * 64 + 16 * jt + r];
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) {
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = H1s[(8 * g + e) * 64 + 16 * kt + r];
+        gW2[jt][kt] = mfma(a, b, gW2[jt][kt]);
+      }
+    }
+    // dH1 = (dH2 W2) * relu'(h1)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const bf16x8 a0 = load8(dH2s + (16 * mt + r) * 64 + 8 * g);
+      const bf16x8 a1 = load8(dH2s + (16 * mt + r) * 64 + 32 + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        f4 c = mfma(a0, w2t[nt][0], f4{0, 0, 0, 0});
+        c = mfma(a1, w2t[nt][1], c);
+        const int col = 16 * nt + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * mt + 4 * g + i;
+          const float v = (float)H1s[row * 64 + col] > 0.0f ? c[i] : 0.0f;
+          gb1[nt] += v;
+          dH1s[row * 64 + col] = (__bf16)v;
+        }
+      }
+    }
+    __syncthreads();
+    // dW1[j][k] += sum_s dH1[s][j] X[s][k]
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      bf16x8 a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = dH1s[(8 * g + e) * 64 + 16 * jt + r];
+#pragma unroll
+      for (int kt = 0; kt < 2; ++kt) {
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = Xs[(8 * g + e) * 32 + 16 * kt + r];
+        gW1[jt][kt] = mfma(a, b, gW1[jt][kt]);
+      }
+    }
+    __syncthreads();
+  }
+  // flush: D tiles hold rows 4g+i (output index within the tile), column r
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int j = 16 * jt + 4 * g + i;
+      for (int kt = 0; kt < 2; ++kt) {
+        const int k = 16 * kt + r;
+        if (k < 22) atomicAdd(&grad[W1_OFF + 22 * j + k], gW1[jt][kt][i]);
+      }
+      for (int kt = 0; kt < 4; ++kt) atomicAdd(&grad[W2_OFF + 64 * j + 16 * kt + r], gW2[jt][kt][i]);
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = 4 * g + i;
+      if (o < 4) atomicAdd(&grad[W3_OFF + 64 * o + 16 * nt + r], gW3[nt][i]);
+    }
+  // bias sums: lanes r, r+16, r+32, r+48 hold partial sums of column r
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    float v1 = gb1[nt], v2 = gb2[nt];
+    v1 += __shfl_xor(v1, 16, 64);
+    v1 += __shfl_xor(v1, 32, 64);
+    v2 += __shfl_xor(v2, 16, 64);
+    v2 += __shfl_xor(v2, 32, 64);
+    if (l < 16) {
+      atomicAdd(&grad[B1_OFF + 16 * nt + l], v1);
+      atomicAdd(&grad[B2_OFF + 16 * nt + l], v2);
+    }
+  }
+  if (l < 4) atomicAdd(&grad[B3_OFF + l], gb3);
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, 64);
+  if (l == 0) atomicAdd(loss, lsum);
+}
+
+// Adam on the FP32 master copy with the mean gradient grad/nsamp; clears grad.
+__global__ void k_mlp_adam(float *__restrict__ P, float *__restrict__ grad, float *__restrict__ m1,
+                           float *__restrict__ m2, int np, float inv_n, float lr, float beta1, float beta2, float eps,
+                           float bc1, float bc2) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    const float g = grad[p] * inv_n;
+    const float a = beta1 * m1[p] + (1.0f - beta1) * g;
+    const float b = beta2 * m2[p] + (1.0f - beta2) * g * g;
+    m1[p] = a;
+    m2[p] = b;
+    P[p] -= lr * (a / bc1) / (sqrtf(b / bc2) + eps);
+    grad[p] = 0.0f;
+  }
+}
+
+__global__ void k_fill_f32(float *p, int n, float v) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) p[t] = v;
+}
+
+// Gathers the six neighbours of site (i,j,k) of an all-active lattice with
+// the lattice's boundary types (bulk neighbourhoods only).
+__device__ __forceinline__ void gather6(const MCDevParams &P, const float *nx, const float *ny, const float *nz,
+                                        int i, int j, int k, float m[6][3], bool on[6]) {
+  const int X = P.nx, Y = P.ny, Z = P.nz;
+  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+  on[0] = on[1] = on[2] = on[3] = on[4] = on[5] = true;
+  if (ip == X) { if (P.per[0]) ip = 0; else on[0] = false; }
+  if (im < 0) { if (P.per[0]) im = X - 1; else on[1] = false; }
+  if (jp == Y) { if (P.per[1]) jp = 0; else on[2] = false; }
+  if (jm < 0) { if (P.per[1]) jm = Y - 1; else on[3] = false; }
+  if (kp == Z) { if (P.per[2]) kp = 0; else on[4] = false; }
+  if (km < 0) { if (P.per[2]) km = Z - 1; else on[5] = false; }
+  const int idx[6] = {ip + X * (j + Y * k), im + X * (j + Y * k), i + X * (jp + Y * k),
+                      i + X * (jm + Y * k), i + X * (j + Y * kp), i + X * (j + Y * km)};
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const int s = on[b] ? idx[b] : 0;
+    m[b][0] = on[b] ? nx[s] : 0.0f;
+    m[b][1] = on[b] ? ny[s] : 0.0f;
+    m[b][2] = on[b] ? nz[s] : 0.0f;
+  }
+}
+
+// Training samples from the current lattice: random sites (Philox keyed by
+// seed, counter = sample index) -> features, base, centre director.
+__global__ void k_mlp_dataset(MCDevParams P, const float *__restrict__ nx, const float *__restrict__ ny,
+                              const float *__restrict__ nz, const MCDevState *st, int n, uint32_t seed,
+                              float *__restrict__ X, float *__restrict__ base, float *__restrict__ dirs) {
+  const float T = st->T, E = st->elecE;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const MCU4 r = mcd_philox(MCU4{(uint32_t)s, seed, P.stream, 0x4D53u}, P.seed_lo, P.seed_hi);
+    const int site = (int)(((uint64_t)r.x * (uint64_t)P.nt) >> 32);
+    const int i = site % P.nx, j = (site / P.nx) % P.ny, k = site / (P.nx * P.ny);
+    float m[6][3];
+    bool on[6];
+    gather6(P, nx, ny, nz, i, j, k, m, on);
+    float x[22], b[3];
+    features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
+#pragma unroll
+    for (int c = 0; c < 22; ++c) X[(size_t)s * 22 + c] = x[c];
+    base[3 * (size_t)s] = b[0];
+    base[3 * (size_t)s + 1] = b[1];
+    base[3 * (size_t)s + 2] = b[2];
+    dirs[3 * (size_t)s] = nx[site];
+    dirs[3 * (size_t)s + 1] = ny[site];
+    dirs[3 * (size_t)s + 2] = nz[site];
+  }
+}
+
+// Draws from the axial vMF mixture (Wood 1994 for the polar cosine on S^2,
+// w = 1 + log(v + (1 - v) e^{-2 kappa}) / kappa with v in (0, 1], a uniform
+// azimuth around mu, then the sign of the mixture component).
+__device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa, float u_w, float u_phi, bool neg,
+                                                 float q[3]) {
+  const float v = 1.0f - u_w;  // (0, 1]
+  float w = 1.0f + __logf(v + (1.0f - v) * __expf(-2.0f * kappa)) / kappa;
+  w = fminf(fmaxf(w, -1.0f), 1.0f);
+  const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  // orthonormal e1, e2 perpendicular to mu
+  const float ax = fabsf(mu[0]), ay = fabsf(mu[1]), az = fabsf(mu[2]);
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  if (ax <= ay && ax <= az) a0 = 1.0f; else if (ay <= az) a1 = 1.0f; else a2 = 1.0f;
+  float e0 = mu[1] * a2 - mu[2] * a1, e1 = mu[2] * a0 - mu[0] * a2, e2 = mu[0] * a1 - mu[1] * a0;
+  const float ie = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  e0 *= ie;
+  e1 *= ie;
+  e2 *= ie;
+  const float f0 = mu[1] * e2 - mu[2] * e1, f1 = mu[2] * e0 - mu[0] * e2, f2 = mu[0] * e1 - mu[1] * e0;
+  const float ph = u_phi;  // revolutions
+  const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+  float q0 = w * mu[0] + st * (cs * e0 + sn * f0);
+  float q1 = w * mu[1] + st * (cs * e1 + sn * f1);
+  float q2 = w * mu[2] + st * (cs * e2 + sn * f2);
+  const float iq = rsqrtf(q0 * q0 + q1 * q1 + q2 * q2);
+  const float sg = neg ? -iq : iq;
+  q[0] = q0 * sg;
+  q[1] = q1 * sg;
+  q[2] = q2 * sg;
+}
+
+// One colour pass of the learned-proposal Metropolis-Hastings sweep over an
+// all-active lattice; one wave per workgroup, 64 sites per iteration.
+template <int POT>
+__global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, float *__restrict__ nx,
+                                                     float *__restrict__ ny, float *__restrict__ nz, MCDevState *st,
+                                                     int colour, int finalize, MCLearnRecord rec) {
+  __shared__ __align__(16) __bf16 Xs[64 * 32];
+  __shared__ __align__(16) __bf16 H1s[64 * 64];
+  __shared__ __align__(16) __bf16 H2s[64 * 64];
+  __shared__ float Os[64 * 16];
+  const float T = st->T, E = st->elecE;
+  const uint32_t sweep = st->sweep;
+  const double cA = (double)(-P.A) * 1.5;
+  const float cAf = (float)cA;
+  bf16x8 w1[4], w2[4][2], w3[2];
+  load_weights(N, w1, w2, w3);
+  const int l = threadIdx.x;
+  const int total = mcd_set_slots(P, 8 + colour);
+  uint32_t acc = 0;
+  for (int blk = blockIdx.x; blk * 64 < total; blk += gridDim.x) {
+    const int t = blk * 64 + l;
+    int i = 0, j = 0, k = 0;
+    bool valid = t < total && mcd_set_site(P, 8 + colour, t, i, j, k);
+    const int s = valid ? i + P.nx * (j + P.ny * k) : 0;
+    float m[6][3];
+    bool on[6];
+    gather6(P, nx, ny, nz, i, j, k, m, on);
+    float x[22], b[3];
+    features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
+    write_xrow(Xs, l, x, valid);
+    __syncthreads();
+    forward64(N, w1, w2, w3, Xs, H1s, H2s, Os);
+    float mu[3], lk;
+    head(Os + l * 16, b, mu, lk);
+    const float kappa = __expf(lk);
+    const MCU4 r = mcd_philox(MCU4{(uint32_t)s, sweep, P.stream, 0x4D4Cu}, P.seed_lo, P.seed_hi);
+    float q[3];
+    sample_axial_vmf(mu, kappa, mcd_u01(r.x), mcd_u01(r.y), (r.z & 1u) != 0u, q);
+    const float o0 = valid ? nx[s] : 1.0f, o1 = valid ? ny[s] : 0.0f, o2 = valid ? nz[s] : 0.0f;
+    float dE;
+    if (POT == MCD_LL) {
+      float a2 = 0.0f;
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb) {
+        if (!on[bb]) continue;
+        const float a = fmaf(o2, m[bb][2], fmaf(o1, m[bb][1], o0 * m[bb][0]));
+        const float c = fmaf(q[2], m[bb][2], fmaf(q[1], m[bb][1], q[0] * m[bb][0]));
+        a2 = fmaf(c - a, c + a, a2);
+      }
+      dE = cAf * a2;
+    } else {
+      double so = 0.0, sn = 0.0;
+#pragma unroll
+      for (int bb = 0; bb < 6; ++bb) {
+        if (!on[bb]) continue;
+        so += (double)mcd_bond<POT>(P, cA, bb, o0, o1, o2, m[bb][0], m[bb][1], m[bb][2]);
+        sn += (double)mcd_bond<POT>(P, cA, bb, q[0], q[1], q[2], m[bb][0], m[bb][1], m[bb][2]);
+      }
+      dE = (float)sn - (float)so;
+    }
+    if (P.elecA != 0.0f) dE += mcd_field(P, E, q[0], q[1], q[2]) - mcd_field(P, E, o0, o1, o2);
+    const float co = mu[0] * o0 + mu[1] * o1 + mu[2] * o2;
+    const float cn = mu[0] * q[0] + mu[1] * q[1] + mu[2] * q[2];
+    const float lr = lcosh2(kappa * co) - lcosh2(kappa * cn);
+    const float xlog = -dE / T + lr;
+    const float u = mcd_u01(r.w);
+    const bool a = valid && u < __expf(fminf(xlog, 80.0f));
+    if (a) {
+      nx[s] = q[0];
+      ny[s] = q[1];
+      nz[s] = q[2];
+    }
+    acc += a ? 1u : 0u;
+    if (rec.mu && valid) {
+      rec.mu[3 * (size_t)s] = mu[0];
+      rec.mu[3 * (size_t)s + 1] = mu[1];
+      rec.mu[3 * (size_t)s + 2] = mu[2];
+      rec.logk[s] = lk;
+      rec.prop[3 * (size_t)s] = q[0];
+      rec.prop[3 * (size_t)s + 1] = q[1];
+      rec.prop[3 * (size_t)s + 2] = q[2];
+      rec.u[s] = u;
+      rec.acc[s] = a ? 1 : 0;
+      rec.dE[s] = dE;
+    }
+    __syncthreads();
+  }
+  // acceptance bookkeeping; ang_var (unused by the learned proposal) is left
+  // as it is for plain sweeps interleaved with learned ones
+  uint32_t v = acc;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (threadIdx.x == 0) {
+    atomicAdd(&st->acc_count, v);
+    if (finalize) {
+      __threadfence();
+      const uint32_t done = atomicAdd(&st->blocks_done, 1u);
+      if (done == gridDim.x - 1) {
+        __threadfence();
+        const uint32_t tot = atomicAdd(&st->acc_count, 0u);
+        st->last_acc = tot;
+        st->acc_count = 0u;
+        st->blocks_done = 0u;
+        st->sweep = st->sweep + 1u;
+      }
+    }
+  }
+}
+
+MLPDev dev_of(const MCMLPBuffers &B) {
+  MLPDev N;
+  N.W1b = B.W1b;
+  N.W2b = B.W2b;
+  N.W3b = B.W3b;
+  N.W3Tb = B.W3Tb;
+  N.W2Tb = B.W2Tb;
+  N.b1 = B.b1;
+  N.b2 = B.b2;
+  N.b3 = B.b3;
+  return N;
+}
+
+}  // namespace
+
+void mc_mlp_launch_pack(const MCMLPBuffers &B, hipStream_t s) {
+  k_mlp_pack<<<16, 256, 0, s>>>(B.params, B.W1b, B.W2b, B.W3b, B.W3Tb, B.W2Tb, B.b1, B.b2, B.b3);
+}
+
+void mc_mlp_launch_infer(const MCMLPBuffers &B, hipStream_t s, const float *X, const float *base, int n, float *mu,
+                         float *logk) {
+  const int grid = (n + 63) / 64 < 4096 ? (n + 63) / 64 : 4096;
+  k_mlp_infer<<<grid > 0 ? grid : 1, 64, 0, s>>>(dev_of(B), X, base, n, mu, logk);
+}
+
+void mc_mlp_launch_grad(const MCMLPBuffers &B, hipStream_t s, const float *X, const float *base, const float *dirs,
+                        int n, int grid) {
+  k_mlp_grad<<<grid, 64, 0, s>>>(dev_of(B), X, base, dirs, n, B.grad, B.loss);
+}
+
+void mc_mlp_launch_adam(const MCMLPBuffers &B, hipStream_t s, int nsamp, float lr, int step) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+  k_mlp_adam<<<24, 256, 0, s>>>(B.params, B.grad, B.m1, B.m2, MC_MLP_NPARAM, 1.0f / nsamp, lr, b1, b2, 1e-8f, bc1,
+                                bc2);
+}
+
+void mc_mlp_launch_fill(float *p, int n, float v, hipStream_t s) { k_fill_f32<<<24, 256, 0, s>>>(p, n, v); }
+
+void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, float *base, float *dirs) {
+  const int grid = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  k_mlp_dataset<<<grid, 256, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, n, seed, X, base, dirs);
+}
+
+void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
+                                const MCLearnRecord *rec) {
+  const MCLearnRecord none{};
+  const MCLearnRecord &R = rec ? *rec : none;
+  switch (L.P.potential) {
+    case MCD_GHRL:
+      k_learned_pass<MCD_GHRL><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      break;
+    case MCD_PEAR:
+      k_learned_pass<MCD_PEAR><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      break;
+    default:
+      k_learned_pass<MCD_LL><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      break;
+  }
+}

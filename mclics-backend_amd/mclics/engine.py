@@ -71,6 +71,18 @@ EXPORTS = (
     "mc_batch_get_state",
     "mc_batch_set_tuning",
     "mc_batch_autotune",
+    "mc_mlp_create",
+    "mc_mlp_destroy",
+    "mc_mlp_get_params",
+    "mc_mlp_set_params",
+    "mc_mlp_infer",
+    "mc_mlp_grad",
+    "mc_mlp_dataset_add",
+    "mc_mlp_dataset_size",
+    "mc_mlp_dataset_clear",
+    "mc_mlp_train",
+    "mc_set_learned_proposal",
+    "mc_record_learned_sweep",
 )
 
 
@@ -170,6 +182,18 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_batch_get_state.argtypes = [vp, u32p, u32p, f32p]
     lib.mc_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     lib.mc_batch_autotune.argtypes = [vp, ip, ip, f32p]
+    lib.mc_mlp_create.argtypes = [f32p, C.c_uint64, C.POINTER(vp)]
+    lib.mc_mlp_destroy.argtypes = [vp]
+    lib.mc_mlp_get_params.argtypes = [vp, f32p]
+    lib.mc_mlp_set_params.argtypes = [vp, f32p]
+    lib.mc_mlp_infer.argtypes = [vp, C.c_int, f32p, f32p, f32p, f32p]
+    lib.mc_mlp_grad.argtypes = [vp, C.c_int, f32p, f32p, f32p, dp, f32p]
+    lib.mc_mlp_dataset_add.argtypes = [vp, vp, C.c_int, C.c_uint32]
+    lib.mc_mlp_dataset_size.argtypes = [vp, ip]
+    lib.mc_mlp_dataset_clear.argtypes = [vp]
+    lib.mc_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, f32p]
+    lib.mc_set_learned_proposal.argtypes = [vp, vp, C.c_int]
+    lib.mc_record_learned_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p]
     for name in EXPORTS:
         if name not in ("mc_last_error", "mc_device_count"):
             getattr(lib, name).restype = C.c_int
@@ -373,6 +397,19 @@ class Lattice:
         _check(self._lib.mc_pass(self._h, set_id, _f32(prop), _f32(uu), _u8(acc), _f32(eo), _f32(en)))
         return acc, eo, en
 
+    def set_learned(self, net: Optional["Network"], every: int = 1) -> None:
+        _check(self._lib.mc_set_learned_proposal(self._h, None if net is None else net._h, 0 if net is None else every))
+        self._net = net
+
+    def record_learned_sweep(self):
+        nt = self.nt
+        r = {"mu": np.zeros(3 * nt, np.float32), "logk": np.zeros(nt, np.float32),
+             "prop": np.zeros(3 * nt, np.float32), "u": np.zeros(nt, np.float32),
+             "accept": np.zeros(nt, np.uint8), "dE": np.zeros(nt, np.float32)}
+        _check(self._lib.mc_record_learned_sweep(self._h, _f32(r["mu"]), _f32(r["logk"]), _f32(r["prop"]),
+                                                 _f32(r["u"]), _u8(r["accept"]), _f32(r["dE"])))
+        return r
+
     def record_sweep(self):
         nt = self.nt
         r = {k: np.zeros(nt, np.float32) for k in ("u0", "u1", "u2", "e_old", "e_new")}
@@ -464,6 +501,81 @@ class Batch:
         g, x, us = C.c_int(), C.c_int(), C.c_float()
         _check(self._lib.mc_batch_autotune(self._h, C.byref(g), C.byref(x), C.byref(us)))
         return g.value, x.value, us.value
+
+
+MLP_NPARAM = 5892
+
+
+class Network:
+    """The learned-proposal network (an mc_mlp)."""
+
+    def __init__(self, params: Optional[np.ndarray] = None, seed: int = 1):
+        lib = load_library()
+        h = C.c_void_p()
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        _check(lib.mc_mlp_create(None if p is None else _f32(p), seed, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.mc_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_NPARAM, np.float32)
+        _check(self._lib.mc_mlp_get_params(self._h, _f32(out)))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        pp = np.ascontiguousarray(p, np.float32)
+        _check(self._lib.mc_mlp_set_params(self._h, _f32(pp)))
+
+    def infer(self, X: np.ndarray, base: np.ndarray):
+        X = np.ascontiguousarray(X, np.float32)
+        base = np.ascontiguousarray(base, np.float32)
+        n = len(X)
+        mu = np.empty((n, 3), np.float32)
+        lk = np.empty(n, np.float32)
+        _check(self._lib.mc_mlp_infer(self._h, n, _f32(X), _f32(base), _f32(mu), _f32(lk)))
+        return mu, lk
+
+    def grad(self, X, base, dirs):
+        X = np.ascontiguousarray(X, np.float32)
+        base = np.ascontiguousarray(base, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        _check(self._lib.mc_mlp_grad(self._h, len(X), _f32(X), _f32(base), _f32(dirs), C.byref(loss), _f32(g)))
+        return loss.value, g
+
+    def add_samples(self, lattice: "Lattice", n: int, seed: int) -> None:
+        _check(self._lib.mc_mlp_dataset_add(self._h, lattice._h, n, seed))
+
+    def dataset_size(self) -> int:
+        v = C.c_int()
+        _check(self._lib.mc_mlp_dataset_size(self._h, C.byref(v)))
+        return v.value
+
+    def clear_samples(self) -> None:
+        _check(self._lib.mc_mlp_dataset_clear(self._h))
+
+    def train(self, epochs: int, batch: int = 1024, lr: float = 1e-3) -> np.ndarray:
+        out = np.empty(epochs, np.float32)
+        _check(self._lib.mc_mlp_train(self._h, epochs, batch, lr, _f32(out)))
+        return out
 
 
 def device_count() -> int:

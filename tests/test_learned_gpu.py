@@ -1,0 +1,183 @@
+"""The learned proposal on the GPU: bf16-MFMA inference and training
+gradients against the FP64 oracle, training on plain-sampler neighbourhoods,
+the exact Metropolis-Hastings decisions of a learned sweep, and agreement of
+E and S between learned and plain sweeps (3 sigma over replicas)."""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import mclics
+
+pytestmark = pytest.mark.gpu
+
+
+def unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def rand_params(rng, scale=0.25):
+    p = (rng.normal(size=O.MLP_NPARAM) * scale).astype(np.float32)
+    return p
+
+
+def feature_batch(rng, n, T=1.1):
+    X, B, D = [], [], []
+    for _ in range(n):
+        x, b = O.mlp_features(unit(rng, 6).reshape(-1), (rng.normal(size=3) * 0.3).astype(np.float32), T)
+        X.append(x)
+        B.append(b)
+        D.append(unit(rng, 1)[0])
+    return np.array(X, np.float32), np.array(B, np.float32), np.array(D, np.float32)
+
+
+def test_inference_matches_oracle():
+    rng = np.random.default_rng(1)
+    P = rand_params(rng)
+    X, B, _ = feature_batch(rng, 300)
+    with mclics.Network(P) as net:
+        mu, lk = net.infer(X, B)
+    Pb = P.copy()
+    nw = 64 * 22 + 64 + 64 * 64 + 64 + 4 * 64 + 4
+    # the device uses bf16 weights, FP32 biases
+    idx_w = np.r_[0:64 * 22, 64 * 22 + 64:64 * 22 + 64 + 64 * 64, 64 * 22 + 128 + 64 * 64:nw - 4]
+    Pb[idx_w] = O.bf16(P[idx_w])
+    err_mu, err_lk = [], []
+    for s in range(len(X)):
+        out = O.mlp_forward(Pb, X[s], 1)
+        m, l = O.mlp_head(out, B[s])
+        err_mu.append(np.abs(mu[s] - m).max())
+        err_lk.append(abs(lk[s] - l))
+    assert np.median(err_mu) < 2e-3 and max(err_mu) < 5e-2
+    assert np.median(err_lk) < 2e-3 and max(err_lk) < 5e-2
+
+
+def test_gradient_matches_oracle():
+    rng = np.random.default_rng(2)
+    P = rand_params(rng, 0.2)
+    X, B, D = feature_batch(rng, 256)
+    with mclics.Network(P) as net:
+        loss, g = net.grad(X, B, D)
+    nw = O.MLP_NPARAM
+    Pb = P.copy()
+    idx_w = np.r_[0:64 * 22, 64 * 22 + 64:64 * 22 + 64 + 64 * 64, 64 * 22 + 128 + 64 * 64:nw - 4]
+    Pb[idx_w] = O.bf16(P[idx_w])
+    lo, go = O.mlp_nll_grad(Pb, X, B, D, 1)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    cos = float(np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)))
+    assert cos > 0.995, cos
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 0.1
+    # each layer's block agrees on its own
+    for a, b in ((0, 64 * 22), (64 * 22, 64 * 22 + 64), (64 * 22 + 64, 64 * 22 + 64 + 4096),
+                 (64 * 22 + 128 + 4096, nw)):
+        ga, gb = g[a:b], go[a:b]
+        assert np.dot(ga, gb) / (np.linalg.norm(ga) * np.linalg.norm(gb) + 1e-30) > 0.99
+
+
+def equilibrated(L, T, sweeps, seed):
+    lat = mclics.Lattice(mclics.Config(L, L, L, T=T, seed=seed), mclics.random_ic(L, L, L, seed=seed))
+    lat.sweeps(sweeps)
+    lat.sync()
+    return lat
+
+
+def test_training_lowers_the_loss_and_raises_acceptance():
+    L = 16
+    with mclics.Network(seed=3) as net:
+        for T, seed in ((1.05, 11), (1.1, 12), (1.2, 13)):
+            lat = equilibrated(L, T, 300, seed)
+            for k in range(4):
+                lat.sweeps(20)
+                net.add_samples(lat, 8192, seed=100 * seed + k)
+            lat.close()
+        assert net.dataset_size() == 3 * 4 * 8192
+        lat = equilibrated(L, 1.1, 300, 21)
+        lat.set_learned(net)
+        acc0 = lat.record_learned_sweep()["accept"].mean()
+        losses = net.train(epochs=6, batch=1024, lr=2e-3)
+        assert losses[-1] < losses[0] - 0.05, losses
+        acc1 = lat.record_learned_sweep()["accept"].mean()
+        lat.close()
+    assert acc1 > acc0 + 0.05, (acc0, acc1)
+
+
+def test_learned_sweep_decisions_are_exact_mh():
+    # Replays one recorded learned sweep: the oracle recomputes, in FP64, the
+    # energy change and the proposal-density ratio from the recorded mu/kappa
+    # and must reach the same decision for every move that is not a near tie.
+    L, T = 12, 1.1
+    rng = np.random.default_rng(4)
+    P = rand_params(rng, 0.1)
+    po = O.params(L, L, L)
+    pt = np.ones(L**3, np.int32)
+    with mclics.Network(P) as net, mclics.Lattice(mclics.Config(L, L, L, T=T, seed=5),
+                                                  mclics.random_ic(L, L, L, seed=5)) as lat:
+        lat.sweeps(50)
+        start = lat.directors()
+        lat.set_learned(net)
+        rec = lat.record_learned_sweep()
+        end = lat.directors()
+    state = start.copy().reshape(-1, 3)
+    near, checked = 0, 0
+    for colour in (0, 1):
+        sites = [i + L * (j + L * k) for k in range(L) for j in range(L) for i in range(L) if (i + j + k) % 2 == colour]
+        flat = state.reshape(-1).copy()
+        for s in sites:
+            i, j, k = s % L, (s // L) % L, s // (L * L)
+            q = rec["prop"][3 * s:3 * s + 3]
+            assert abs(np.linalg.norm(q.astype(np.float64)) - 1) < 1e-5
+            import ctypes as C
+            eo = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, None)
+            en = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, O.f32(q.copy()))
+            dE = float(en) - float(eo)
+            assert rec["dE"][s] == pytest.approx(dE, abs=2e-5)
+            mu = rec["mu"][3 * s:3 * s + 3].astype(np.float64)
+            kap = math.exp(float(rec["logk"][s]))
+            o = flat[3 * s:3 * s + 3].astype(np.float64)
+            x = -dE / T + (O.vmf_logq(mu, kap, o) - O.vmf_logq(mu, kap, q.astype(np.float64)))
+            u = float(rec["u"][s])
+            if u > 0 and abs(math.log(u) - x) < 1e-3:
+                near += 1
+                acc = bool(rec["accept"][s])
+            else:
+                acc = u < math.exp(min(x, 700.0))
+                assert acc == bool(rec["accept"][s]), (colour, s, x, u)
+            checked += 1
+            if acc:
+                state[s] = q
+        # next colour sees this colour's accepted moves
+    assert near < 0.01 * checked
+    np.testing.assert_array_equal(state.reshape(-1), end)
+
+
+@pytest.mark.slow
+def test_learned_and_plain_agree_within_3_sigma():
+    # Same lattice, temperatures on both sides of the transition; R replicas
+    # per method: |mean_A - mean_B| <= 3 sqrt(se_A^2 + se_B^2).
+    L, R, nequil, nmeas = 8, 8, 400, 1600
+    with mclics.Network(seed=6) as net:
+        lat = equilibrated(16, 1.1, 300, 31)
+        for k in range(3):
+            lat.sweeps(20)
+            net.add_samples(lat, 8192, seed=700 + k)
+        lat.close()
+        net.train(epochs=4, batch=1024, lr=2e-3)
+        for T in (0.9, 1.3):
+            res = {}
+            for method in ("plain", "learned"):
+                Em, Sm = [], []
+                for r in range(R):
+                    seed = 1000 + r + (0 if method == "plain" else 500)
+                    with mclics.Lattice(mclics.Config(L, L, L, T=T, seed=seed), mclics.random_ic(L, L, L, seed=seed)) as lt:
+                        if method == "learned":
+                            lt.set_learned(net)
+                        lt.sweeps(nequil)
+                        E, S = lt.measure_series(nmeas)
+                    Em.append(E.mean())
+                    Sm.append(S.mean())
+                res[method] = (np.mean(Em), np.std(Em, ddof=1) / math.sqrt(R), np.mean(Sm), np.std(Sm, ddof=1) / math.sqrt(R))
+            (Ea, ea, Sa, sa), (Eb, eb, Sb, sb) = res["plain"], res["learned"]
+            assert abs(Ea - Eb) <= 3 * math.hypot(ea, eb), (T, res)
+            assert abs(Sa - Sb) <= 3 * math.hypot(sa, sb), (T, res)
