@@ -156,6 +156,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--learned-sweeps", type=int, default=30000,
                     help="measured sweeps per chain in the learned-proposal leg")
+    ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
+                    help="sweep kernel of the headline leg (auto: slab where it fits)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the device-copy calibration for the HBM byte counters")
     args = ap.parse_args()
@@ -181,6 +183,8 @@ def main():
     per_t = [args.steps // len(TEMPS) + (1 if i < args.steps % len(TEMPS) else 0) for i in range(len(TEMPS))]
     obs = []
     with mclics.Lattice(cfg, n0) as lat:
+        lat.set_kernel({"auto": mclics.KERNEL_AUTO, "slab": mclics.KERNEL_SLAB, "x4": mclics.KERNEL_X4,
+                        "generic": mclics.KERNEL_GENERIC}[args.kernel])
         log(f"bench: warmup {args.warmup} sweeps")
         if args.warmup:
             lat.sweeps(args.warmup)
@@ -198,12 +202,17 @@ def main():
             E, S, _ = lat.measure()
             obs.append({"T": T, "E": E, "S": S})
         acc, sweeps = lat.acceptance()
-        kname = "k_sweep_x4 (segment kernel, one colour pass)" if lat.kernel() == mclics.KERNEL_X4 else \
-            "k_sweep_pass (site kernel, one colour pass)"
+        kind = lat.kernel()
+        kname = {mclics.KERNEL_SLAB: "k_sweep_slab (slab kernel, both colours, one launch per sweep)",
+                 mclics.KERNEL_X4: "k_sweep_x4 (segment kernel, one colour pass)"}.get(
+            kind, "k_sweep_pass (site kernel, one colour pass)")
+        tile = lat.slab_tile() if kind == mclics.KERNEL_SLAB else None
         log(f"bench: {args.steps} sweeps in {wall:.3f} s, mean pass {pass_ms_sum / passes * 1e3:.2f} us")
     value = nt * args.steps / wall
     avg_pass_s = pass_ms_sum / passes / 1e3
-    bytes_per_launch = BYTES_PER_SITE_UPDATE * nt / 2  # a pass updates one colour
+    # a launch updates one colour (two-pass kernels) or the whole lattice (slab)
+    launches_per_sweep = passes / args.steps
+    bytes_per_launch = BYTES_PER_SITE_UPDATE * nt / launches_per_sweep
     achieved = bytes_per_launch / avg_pass_s / 1e9
     result = {
         "metric": METRIC,
@@ -224,7 +233,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": kname, "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": avg_pass_s * 1e6},
+                     "avg_launch_us": avg_pass_s * 1e6, "launches_per_sweep": launches_per_sweep,
+                     "slab_tile": tile},
         "observables": obs,
         "last_acceptance": acc / nt,
     }

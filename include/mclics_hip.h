@@ -48,12 +48,15 @@ enum mc_anchoring {
  * (src/evolve.cpp:100-102); MC_ORDER_CHECKER updates the two (i+j+k) parity
  * colours, a different but equally valid Markov chain. */
 enum mc_order { MC_ORDER_CHECKER = 0, MC_ORDER_REF8 = 1 };
-/* Sweep kernels.  AUTO picks the segment kernel (X4) whenever the lattice is
- * all active, periodic on every axis, has Nx % 4 == 0 and uses the
- * checkerboard order, else the generic site kernel.  The two use different
- * counter layouts of the generator, so they are different (equally valid)
- * chains from the same seed. */
-enum mc_kernel { MC_KERNEL_AUTO = 0, MC_KERNEL_GENERIC = 1, MC_KERNEL_X4 = 2 };
+/* Sweep kernels.  The segment kernel (X4, one launch per colour) and the slab
+ * kernel (SLAB, both colours in one launch that streams the lattice once) need
+ * an all-active lattice, periodic on every axis, with Nx % 4 == 0 and the
+ * checkerboard order; AUTO picks SLAB when a tile shape fits the device, else
+ * X4, else the generic site kernel.  X4 and SLAB draw the same numbers and
+ * evolve the lattice identically; the generic kernel uses another counter
+ * layout of the generator, so it is a different (equally valid) chain from
+ * the same seed. */
+enum mc_kernel { MC_KERNEL_AUTO = 0, MC_KERNEL_GENERIC = 1, MC_KERNEL_X4 = 2, MC_KERNEL_SLAB = 3 };
 
 /* POD flattening of the reference's Parameters (include/parameters.h:11-83)
  * restricted to what the sweep reads.  The std::map surface tables become
@@ -106,9 +109,17 @@ int mc_get_sweep_kernel(mc_handle *h, int *kind);
  * workgroups (0 = 8 per CU), xcd = 1 gives each of the 8 XCDs a contiguous
  * eighth of the tiles (grid must then be a multiple of 8). */
 int mc_set_tuning(mc_handle *h, int grid, int xcd);
-/* Times the candidate launch shapes on a scratch copy of the lattice (the
- * chain itself is not advanced), keeps the fastest and reports it with its
- * mean sweep time in microseconds (any output may be NULL). */
+/* Tile shape of the slab kernel: ty full rows (Ny % ty == 0) of cz planes
+ * (Nz % cz == 0) per workgroup, xcd = 1 keeps the tiles of one group of
+ * planes on one XCD.  0 for ty or cz restores the default shape.  Rejected
+ * (MC_EINVAL) when the shape does not fit the device. */
+int mc_set_slab_tile(mc_handle *h, int ty, int cz, int xcd);
+int mc_get_slab_tile(mc_handle *h, int *ty, int *cz, int *xcd);
+/* Times the candidate launch shapes of the kernel in use (segment: grid and
+ * xcd; slab: tile shape, reported as grid = number of workgroups) on a
+ * scratch copy of the lattice (the chain itself is not advanced), keeps the
+ * fastest and reports it with its mean sweep time in microseconds (any output
+ * may be NULL). */
 int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us);
 
 /* n full sweeps (EvolveNGPU::Monte_Carlo_Step_GPU, src/evolve.cu:265-388),

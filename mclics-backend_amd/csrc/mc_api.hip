@@ -70,14 +70,21 @@ struct mc_handle {
   hipStream_t stream = nullptr;
   MCDevParams P{};
   int order = MC_ORDER_CHECKER;
-  int kernel = MC_KERNEL_AUTO;  // sweep kernel selection
+  int kernel = MC_KERNEL_AUTO;  // sweep kernel selection as requested
+  int active = MC_KERNEL_GENERIC;  // kernel in use (AUTO resolved)
   int x4_grid = 0;              // workgroups of a segment pass (0: 8 per CU)
   int x4_xcd = 0;               // tile schedule of the segment pass
   mc_mlp *learned = nullptr;    // learned proposal network (null: plain sweeps)
   int learned_every = 0;        // every k-th sweep uses the learned proposal
   long long host_sweeps = 0;    // sweeps enqueued so far (selects learned sweeps)
   int cu_count = 256;
-  float *nx = nullptr, *ny = nullptr, *nz = nullptr;
+  float *nx = nullptr, *ny = nullptr, *nz = nullptr;  // current lattice
+  // Second plane set of the slab kernel, which reads one set and writes the
+  // other; flip tells which allocation holds the current lattice.
+  float *bx = nullptr, *by = nullptr, *bz = nullptr;
+  int flip = 0;
+  MCSlabShape slab{0, 0, 1};  // ty = 0: default shape
+  size_t lds_limit = 65536;
   uint8_t *pt = nullptr;
   float *sx = nullptr, *sy = nullptr, *sz = nullptr;
   MCDevState *st = nullptr;
@@ -122,8 +129,8 @@ void free_handle(mc_handle *h) {
   if (!h) return;
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
-  void *ptrs[] = {h->nx, h->ny, h->nz, h->pt, h->sx, h->sy, h->sz, h->st, h->partials, h->result, h->staging,
-                  h->pbuf, h->abuf, h->series};
+  void *ptrs[] = {h->nx, h->ny, h->nz, h->bx, h->by, h->bz, h->pt, h->sx, h->sy, h->sz, h->st, h->partials,
+                  h->result, h->staging, h->pbuf, h->abuf, h->series};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -138,15 +145,89 @@ int check_handle(mc_handle *h) {
   return MC_OK;
 }
 
-bool use_x4(const mc_handle *h) {
-  if (h->order != MC_ORDER_CHECKER || h->kernel == MC_KERNEL_GENERIC) return false;
-  return mc_x4_eligible(h->launch());
+bool fast_eligible(const mc_handle *h) { return h->order == MC_ORDER_CHECKER && mc_x4_eligible(h->launch()); }
+
+// Default slab tile: the preferred row count that fits, then planes per tile
+// halved until there are at least as many tiles as CUs (cz >= 4).
+bool slab_default(const mc_handle *h, MCSlabShape *out) {
+  const MCDevParams &P = h->P;
+  const int prefs[] = {8, 4, 16, 6, 2, 12, 10, 14, 1};
+  for (int ty : prefs) {
+    MCSlabShape s{ty, P.nz, 1};
+    if (!mc_slab_shape_ok(P, s, h->lds_limit)) continue;
+    while (s.cz % 2 == 0 && s.cz / 2 >= 4 && (long long)(P.ny / ty) * (P.nz / s.cz) < h->cu_count) s.cz /= 2;
+    *out = s;
+    return true;
+  }
+  return false;
 }
 
-int passes_per_sweep(const mc_handle *h) { return h->order == MC_ORDER_REF8 ? 8 : 2; }
+MCSlabShape slab_shape(const mc_handle *h) {
+  if (h->slab.ty > 0) return h->slab;
+  MCSlabShape s{1, 1, 1};
+  slab_default(h, &s);
+  s.xcd = h->slab.xcd;
+  return s;
+}
+
+bool use_x4(const mc_handle *h) { return h->active == MC_KERNEL_X4; }
+bool use_slab(const mc_handle *h) { return h->active == MC_KERNEL_SLAB; }
 
 bool learned_sweep(const mc_handle *h) {
   return h->learned && h->learned_every > 0 && (h->host_sweeps % h->learned_every) == 0;
+}
+
+int passes_per_sweep(const mc_handle *h) {
+  if (h->order == MC_ORDER_REF8) return 8;
+  return use_slab(h) && !learned_sweep(h) ? 1 : 2;
+}
+
+void swap_planes(mc_handle *h) {
+  std::swap(h->nx, h->bx);
+  std::swap(h->ny, h->by);
+  std::swap(h->nz, h->bz);
+  h->flip ^= 1;
+}
+
+void drop_graphs(mc_handle *h) {
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+}
+
+// Resolves the requested kernel and allocates what it needs.
+int select_kernel(mc_handle *h, int kind) {
+  int want = MC_KERNEL_GENERIC;
+  MCSlabShape s;
+  const bool slab_ok = fast_eligible(h) && (h->slab.ty > 0 || slab_default(h, &s));
+  if (kind == MC_KERNEL_SLAB || (kind == MC_KERNEL_AUTO && slab_ok)) want = MC_KERNEL_SLAB;
+  else if (kind == MC_KERNEL_X4 || (kind == MC_KERNEL_AUTO && fast_eligible(h))) want = MC_KERNEL_X4;
+  if (want == MC_KERNEL_SLAB) {
+    if (!slab_ok) return fail(MC_EINVAL, "the slab kernel needs a checkerboard order and an all-active lattice, "
+                                         "periodic on every axis, with Nx a multiple of 4 and a tile that fits");
+    (void)mc_slab_prepare(h->lds_limit);
+  }
+  if (want == MC_KERNEL_SLAB && !h->bx) {
+    const size_t fbytes = (size_t)h->P.nt * sizeof(float);
+    float *b[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < 3; ++d) {
+      if (hipMalloc(&b[d], fbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        for (int e = 0; e < 3; ++e)
+          if (b[e]) (void)hipFree(b[e]);
+        if (kind == MC_KERNEL_SLAB) return fail(MC_ENOMEM, "slab kernel: second plane set allocation failed");
+        want = MC_KERNEL_X4;
+        break;
+      }
+    }
+    if (want == MC_KERNEL_SLAB) {
+      h->bx = b[0];
+      h->by = b[1];
+      h->bz = b[2];
+    }
+  }
+  h->kernel = kind;
+  h->active = want;
+  return MC_OK;
 }
 
 // Pass t (0-based) of one sweep; the last pass adapts ang_var.
@@ -159,6 +240,9 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
     int grid = (slots + 63) / 64;
     if (grid > h->cu_count * 16) grid = h->cu_count * 16;
     mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
+  } else if (use_slab(h)) {
+    mc_launch_sweep_slab(L, h->bx, h->by, h->bz, slab_shape(h), rec);
+    swap_planes(h);
   } else if (use_x4(h)) {
     const int g = h->x4_grid > 0 ? h->x4_grid : h->cu_count * 8;
     mc_launch_sweep_x4(L, t, g, finalize, h->x4_xcd, rec);
@@ -169,9 +253,52 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
 }
 
 int enqueue_sweep(mc_handle *h, const MCRecord *rec, const MCLearnRecord *lrec = nullptr) {
-  for (int t = 0; t < passes_per_sweep(h); ++t) launch_pass(h, t, rec, lrec);
+  const int per = passes_per_sweep(h);
+  for (int t = 0; t < per; ++t) launch_pass(h, t, rec, lrec);
   ++h->host_sweeps;
   MC_LAUNCHED("sweep pass");
+  return MC_OK;
+}
+
+// Graph of n plain sweeps starting from the current plane set.  Capture
+// records the launches without running them, so the host-side plane swaps
+// and sweep count made while capturing are undone afterwards.
+int sweep_graph(mc_handle *h, int n, hipGraphExec_t *out) {
+  const int key = 2 * n + h->flip;
+  auto it = h->graphs.find(key);
+  if (it != h->graphs.end()) {
+    *out = it->second;
+    return MC_OK;
+  }
+  const int flip0 = h->flip;
+  const long long sweeps0 = h->host_sweeps;
+  hipGraph_t g = nullptr;
+  MC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = MC_OK;
+  for (int w = 0; w < n && rc == MC_OK; ++w) rc = enqueue_sweep(h, nullptr);
+  if (h->flip != flip0) swap_planes(h);
+  h->host_sweeps = sweeps0;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc != MC_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != MC_OK) return rc;
+    MC_HIP(e);
+  }
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MC_HIP(e);
+  h->graphs.emplace(key, ge);
+  *out = ge;
+  return MC_OK;
+}
+
+// Replays the graph of n sweeps and advances the host-side bookkeeping the
+// way the captured launches advance the device.
+int launch_sweep_graph(mc_handle *h, int n, hipGraphExec_t ge) {
+  MC_HIP(hipGraphLaunch(ge, h->stream));
+  if (use_slab(h) && (n & 1)) swap_planes(h);
+  h->host_sweeps += n;
   return MC_OK;
 }
 
@@ -307,6 +434,12 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
     return bail(rc);
   }
   h->cu_count = prop.multiProcessorCount;
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) == hipSuccess && lds > 0)
+      h->lds_limit = (size_t)lds;
+    (void)hipGetLastError();
+  }
   MC_HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   MC_HIPC(hipEventCreate(&h->ev0));
   MC_HIPC(hipEventCreate(&h->ev1));
@@ -351,6 +484,8 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   k_init_state<<<1, 1, 0, h->stream>>>(h->st, p->T, p->elecE, p->ang_var);
   MC_HIPC(hipGetLastError());
   MC_HIPC(hipStreamSynchronize(h->stream));
+  rc = select_kernel(h, MC_KERNEL_AUTO);
+  if (rc) return bail(rc);
 #undef MC_HIPC
   *out = h;
   return MC_OK;
@@ -410,23 +545,48 @@ int mc_get_acceptance(mc_handle *h, uint32_t *accepted, uint32_t *sweeps_done) {
 int mc_set_sweep_kernel(mc_handle *h, int kind) {
   int rc = check_handle(h);
   if (rc) return rc;
-  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_GENERIC && kind != MC_KERNEL_X4)
+  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_GENERIC && kind != MC_KERNEL_X4 && kind != MC_KERNEL_SLAB)
     return fail(MC_EINVAL, "mc_set_sweep_kernel: unknown kernel kind");
-  if (kind == MC_KERNEL_X4 && (h->order != MC_ORDER_CHECKER || !mc_x4_eligible(h->launch())))
-    return fail(MC_EINVAL, "mc_set_sweep_kernel: the segment kernel needs a checkerboard order and an all-active "
-                           "lattice, periodic on every axis, with Nx a multiple of 4");
+  if ((kind == MC_KERNEL_X4 || kind == MC_KERNEL_SLAB) && !fast_eligible(h))
+    return fail(MC_EINVAL, "mc_set_sweep_kernel: the segment and slab kernels need a checkerboard order and an "
+                           "all-active lattice, periodic on every axis, with Nx a multiple of 4");
   MC_HIP(hipStreamSynchronize(h->stream));
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
-  h->kernel = kind;
-  return MC_OK;
+  drop_graphs(h);
+  return select_kernel(h, kind);
 }
 
 int mc_get_sweep_kernel(mc_handle *h, int *kind) {
   int rc = check_handle(h);
   if (rc) return rc;
   if (!kind) return fail(MC_EINVAL, "mc_get_sweep_kernel: null output");
-  *kind = use_x4(h) ? MC_KERNEL_X4 : MC_KERNEL_GENERIC;
+  *kind = h->active;
+  return MC_OK;
+}
+
+int mc_set_slab_tile(mc_handle *h, int ty, int cz, int xcd) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (xcd != 0 && xcd != 1) return fail(MC_EINVAL, "mc_set_slab_tile: xcd must be 0 or 1");
+  MCSlabShape s{0, 0, xcd};
+  if (ty != 0 || cz != 0) {
+    s = MCSlabShape{ty, cz, xcd};
+    if (!mc_slab_shape_ok(h->P, s, h->lds_limit))
+      return fail(MC_EINVAL, "mc_set_slab_tile: ty must divide Ny and cz divide Nz, with (Nx/4)*(ty+4) <= 768 "
+                             "threads and the tile's LDS within the device limit");
+  }
+  MC_HIP(hipStreamSynchronize(h->stream));
+  drop_graphs(h);
+  h->slab = s;
+  return MC_OK;
+}
+
+int mc_get_slab_tile(mc_handle *h, int *ty, int *cz, int *xcd) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  const MCSlabShape s = slab_shape(h);
+  if (ty) *ty = s.ty;
+  if (cz) *cz = s.cz;
+  if (xcd) *xcd = s.xcd;
   return MC_OK;
 }
 
@@ -436,16 +596,87 @@ int mc_set_tuning(mc_handle *h, int grid, int xcd) {
   if (grid < 0 || (xcd && grid % 8) || (xcd != 0 && xcd != 1))
     return fail(MC_EINVAL, "mc_set_tuning: grid must be >= 0 (0 = default) and a multiple of 8 with xcd = 1");
   MC_HIP(hipStreamSynchronize(h->stream));
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
+  drop_graphs(h);
   h->x4_grid = grid;
   h->x4_xcd = xcd;
+  return MC_OK;
+}
+
+static int autotune_slab(mc_handle *h, int *grid, int *xcd, float *best_us) {
+  // Candidates sweep a scratch copy of the lattice and state (two plane sets,
+  // swapped after each sweep), so tuning leaves the chain untouched.
+  const size_t nt = (size_t)h->P.nt;
+  const MCDevParams &P = h->P;
+  float *scratch = nullptr;
+  MCDevState *sst = nullptr;
+  MC_HIP(hipMalloc(&scratch, 6 * nt * sizeof(float)));
+  if (hipMalloc(&sst, sizeof(MCDevState)) != hipSuccess) {
+    (void)hipFree(scratch);
+    return fail(MC_ENOMEM, "mc_autotune: scratch state allocation failed");
+  }
+  MCSlabShape best_s = slab_shape(h);
+  float best = 1e30f;
+  hipError_t err = hipSuccess;
+  const int tys[] = {1, 2, 4, 6, 8, 10, 12, 16, 20, 24, 32};
+  for (int ty : tys) {
+    for (int cz = P.nz; cz >= 1 && err == hipSuccess; cz = (cz % 2 == 0) ? cz / 2 : 0) {
+      if (cz < 4 && P.nz >= 8) break;
+      for (int xm = 0; xm < 2 && err == hipSuccess; ++xm) {
+        const MCSlabShape cand{ty, cz, xm};
+        if (!mc_slab_shape_ok(P, cand, h->lds_limit)) continue;
+        if (xm && ((P.ny / ty) * (P.nz / cz)) % 8) continue;
+        float *io[2] = {scratch, scratch + 3 * nt};
+        for (int d = 0; d < 3 && err == hipSuccess; ++d) {
+          float *src = d == 0 ? h->nx : (d == 1 ? h->ny : h->nz);
+          err = hipMemcpyAsync(io[0] + d * nt, src, nt * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+        }
+        if (err == hipSuccess) err = hipMemcpyAsync(sst, h->st, sizeof(MCDevState), hipMemcpyDeviceToDevice, h->stream);
+        if (err != hipSuccess) break;
+        MCLaunch L = h->launch();
+        L.st = sst;
+        int cur = 0;
+        auto sweep = [&]() {
+          L.nx = io[cur];
+          L.ny = io[cur] + nt;
+          L.nz = io[cur] + 2 * nt;
+          float *o = io[cur ^ 1];
+          mc_launch_sweep_slab(L, o, o + nt, o + 2 * nt, cand, nullptr);
+          cur ^= 1;
+        };
+        sweep();  // warm-up
+        sweep();
+        const int reps = 6;
+        err = hipEventRecord(h->ev0, h->stream);
+        for (int w = 0; w < reps; ++w) sweep();
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err == hipSuccess) err = hipEventRecord(h->ev1, h->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(h->ev1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        if (err != hipSuccess) break;
+        const float us = ms * 1e3f / reps;
+        if (us < best) {
+          best = us;
+          best_s = cand;
+        }
+      }
+    }
+  }
+  (void)hipFree(scratch);
+  (void)hipFree(sst);
+  MC_HIP(err);
+  int rc = mc_set_slab_tile(h, best_s.ty, best_s.cz, best_s.xcd);
+  if (rc) return rc;
+  if (grid) *grid = (P.ny / best_s.ty) * (P.nz / best_s.cz);
+  if (xcd) *xcd = best_s.xcd;
+  if (best_us) *best_us = best;
   return MC_OK;
 }
 
 int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us) {
   int rc = check_handle(h);
   if (rc) return rc;
+  if (use_slab(h)) return autotune_slab(h, grid, xcd, best_us);
   if (!use_x4(h)) {
     if (grid) *grid = 0;
     if (xcd) *xcd = 0;
@@ -530,27 +761,10 @@ int mc_sweeps_graph(mc_handle *h, int n) {
   if (n < 0) return fail(MC_EINVAL, "mc_sweeps_graph: negative sweep count");
   if (n == 0) return MC_OK;
   if (h->learned) return mc_sweeps(h, n);  // the learned/plain pattern depends on the sweep count
-  auto it = h->graphs.find(n);
-  if (it == h->graphs.end()) {
-    hipGraph_t g = nullptr;
-    MC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int w = 0; w < n; ++w) {
-      rc = enqueue_sweep(h, nullptr);
-      if (rc) {
-        (void)hipStreamEndCapture(h->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-    }
-    MC_HIP(hipStreamEndCapture(h->stream, &g));
-    hipGraphExec_t ge = nullptr;
-    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MC_HIP(e);
-    it = h->graphs.emplace(n, ge).first;
-  }
-  MC_HIP(hipGraphLaunch(it->second, h->stream));
-  return MC_OK;
+  hipGraphExec_t ge = nullptr;
+  rc = sweep_graph(h, n, &ge);
+  if (rc) return rc;
+  return launch_sweep_graph(h, n, ge);
 }
 
 int mc_sync(mc_handle *h) {
@@ -565,35 +779,21 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   if (rc) return rc;
   if (n < 1 || !ms) return fail(MC_EINVAL, "mc_time_sweeps: need n >= 1 and an output");
   if (use_graph && h->learned) use_graph = 0;
+  hipGraphExec_t ge = nullptr;
   if (use_graph) {
     // Instantiate before timing so capture cost stays outside the region.
-    if (h->graphs.find(n) == h->graphs.end()) {
-      MC_HIP(hipStreamSynchronize(h->stream));
-      hipGraph_t g = nullptr;
-      MC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      for (int w = 0; w < n; ++w) {
-        rc = enqueue_sweep(h, nullptr);
-        if (rc) {
-          (void)hipStreamEndCapture(h->stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          return rc;
-        }
-      }
-      MC_HIP(hipStreamEndCapture(h->stream, &g));
-      hipGraphExec_t ge = nullptr;
-      hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      MC_HIP(e);
-      h->graphs.emplace(n, ge);
-    }
+    MC_HIP(hipStreamSynchronize(h->stream));
+    rc = sweep_graph(h, n, &ge);
+    if (rc) return rc;
   }
+  const int per = passes_per_sweep(h);
   MC_HIP(hipEventRecord(h->ev0, h->stream));
-  rc = use_graph ? mc_sweeps_graph(h, n) : mc_sweeps(h, n);
+  rc = use_graph ? launch_sweep_graph(h, n, ge) : mc_sweeps(h, n);
   if (rc) return rc;
   MC_HIP(hipEventRecord(h->ev1, h->stream));
   MC_HIP(hipEventSynchronize(h->ev1));
   MC_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1));
-  if (launches) *launches = n * passes_per_sweep(h);
+  if (launches) *launches = n * per;
   return MC_OK;
 }
 
@@ -601,17 +801,21 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
   int rc = check_handle(h);
   if (rc) return rc;
   if (n < 1 || !avg_pass_ms) return fail(MC_EINVAL, "mc_time_passes: need n >= 1 and an output");
-  const int per = passes_per_sweep(h);
-  const int np = n * per;
+  int np = 0;
+  const long long sweeps0 = h->host_sweeps;
+  for (int w = 0; w < n; ++w, ++h->host_sweeps) np += passes_per_sweep(h);  // the loop below, dry
+  h->host_sweeps = sweeps0;
   std::vector<hipEvent_t> ev(2 * (size_t)np, nullptr);
   for (auto &e : ev) MC_HIP(hipEventCreate(&e));
   MC_HIP(hipEventRecord(h->ev0, h->stream));
   for (int w = 0, q = 0; w < n; ++w) {
+    const int per = passes_per_sweep(h);
     for (int t = 0; t < per; ++t, ++q) {
       MC_HIP(hipEventRecord(ev[2 * q], h->stream));
       launch_pass(h, t, nullptr);
       MC_HIP(hipEventRecord(ev[2 * q + 1], h->stream));
     }
+    ++h->host_sweeps;
   }
   MC_LAUNCHED("timed sweep pass");
   MC_HIP(hipEventRecord(h->ev1, h->stream));
@@ -657,11 +861,11 @@ int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series) {
     MC_HIP(hipMalloc(&h->series, 2 * (size_t)n * sizeof(double)));
     h->series_cap = n;
   }
-  const MCLaunch L = h->launch();
   for (int w = 0; w < n; ++w) {
     rc = enqueue_sweep(h, nullptr);
     if (rc) return rc;
-    mc_launch_measure(L, h->measure_grid, h->partials, h->result, h->series + 2 * (size_t)w);
+    // (re-read after the sweep: the slab kernel moves the lattice to the other plane set)
+    mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result, h->series + 2 * (size_t)w);
   }
   MC_LAUNCHED("measured sweep");
   std::vector<double> buf(2 * (size_t)n);

@@ -43,6 +43,17 @@ void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *pr
 bool mc_x4_eligible(const MCLaunch &L);
 int mc_x4_slots(const MCDevParams &P);
 void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec);
+// Slab kernel (mc_sweep_slab.hip): one launch per sweep, both colours, reading
+// the lattice from L.nx.. and writing it to the o* planes.  Same eligibility
+// as the segment kernel; a tile is ty full rows of cz planes.
+struct MCSlabShape {
+  int ty, cz, xcd;
+};
+size_t mc_slab_lds_bytes(const MCDevParams &P, int ty);
+bool mc_slab_shape_ok(const MCDevParams &P, const MCSlabShape &s, size_t lds_limit);
+int mc_slab_prepare(size_t lds_bytes);  // number of instantiations that refused the LDS attribute
+void mc_launch_sweep_slab(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabShape &s,
+                          const MCRecord *rec);
 // Batched replicas (mc_sweep_x4.hip / mc_batch.hip).
 struct MCBatchLaunch {
   hipStream_t stream;

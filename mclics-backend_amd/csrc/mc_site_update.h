@@ -1,0 +1,219 @@
+// Two-site Metropolis update shared by the segment kernel (one colour per
+// launch) and the slab kernel (both colours per launch).
+//
+// A thread owns a 16-byte x segment [4q, 4q+4) of one row; the two sites of
+// the colour being updated sit at elements PAR and PAR+2.  The caller gathers
+// the six neighbours of both sites (bond order +x, -x, +y, -y, +z, -z, the
+// order of src/evolve.cpp:128-142) and this header proposes, decides and
+// returns the two decisions.
+//
+// Randomness.  One Philox4x32-10 block per pair (counter = (index of the first
+// site, sweep, stream, 0x4D44)): words 0/2 give the angle uniforms, words 1/3
+// the accept uniforms (top 24 bits each) and the low bytes of the four words
+// the two 16-bit axis uniforms.  The counter depends on the site and the sweep
+// only, so a workgroup that recomputes a neighbour's update (the slab kernel's
+// halo) reaches the same proposal and decision.
+//
+// Accept decision.  For Lebwohl-Lasher the energy difference is first formed
+// in FP32 with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2), whose distance from the
+// reference's own FP32/FP64 arithmetic is bounded by 2^-14*(1.5|A| + |field|)
+// (six bonds, a few roundings each, with a 4x margin).  A hardware exp2 then
+// decides every move whose uniform lies outside that error band around
+// exp(-dE/T); moves inside the band (about 1e-4 of them at T ~ 1) are decided
+// with the exact reference arithmetic, so every decision equals the one the
+// CPU path makes.  GHRL and pear always use the exact arithmetic.
+#pragma once
+#include "mc_kernels.h"
+
+namespace mcsu {
+
+__device__ __forceinline__ float sel(const float4 &v, int e) {
+  return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+
+// 1 accept, 0 reject, -1 inside the error band (decide exactly).
+// invT = RN(1/T) and epsT = RN(eps/T) come from the pass constants.  The
+// screen's x = -dE*invT differs from -dE/T by at most |x| 2^-23; with the
+// 2^-24 |x| of x*log2(e) that is below the |x| 2^-20 term kept in the band,
+// and the hardware exp2's own error is below the 2^-19 term.
+__device__ __forceinline__ int fast_decide(float dE, float invT, float u, float epsT) {
+  if (u == 0.0f) return -1;
+  const float x = -dE * invT;
+  const float rel = epsT + fabsf(x) * 9.53674316e-7f + 1.9073486e-6f;  // + |x| 2^-20 + 2^-19
+  if (!(rel < 0.125f)) return -1;
+  const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  if (fabsf(u - p) <= p * rel) return -1;
+  return u < p ? 1 : 0;
+}
+
+struct Site {
+  float o[3];     // current director
+  float q[3];     // proposal
+  float m[6][3];  // neighbours +x,-x,+y,-y,+z,-z
+  float u;        // accept uniform
+};
+
+template <int POT>
+__device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, float elecE, const Site &S,
+                                               float &Eo, float &En) {
+  double so = 0.0, sn = 0.0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    so += (double)mcd_bond<POT>(P, cA, b, S.o[0], S.o[1], S.o[2], S.m[b][0], S.m[b][1], S.m[b][2]);
+    sn += (double)mcd_bond<POT>(P, cA, b, S.q[0], S.q[1], S.q[2], S.m[b][0], S.m[b][1], S.m[b][2]);
+  }
+  Eo = (float)so;
+  En = (float)sn;
+  if (P.elecA != 0.0f) {
+    Eo += mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
+    En += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]);
+  }
+}
+
+template <int POT>
+__device__ __forceinline__ bool decide(const MCDevParams &P, double cA, float cAf, float epsT, float T, float invT,
+                                       float elecE, const Site &S) {
+  if (POT == MCD_LL) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const float a = fmaf(S.o[2], S.m[b][2], fmaf(S.o[1], S.m[b][1], S.o[0] * S.m[b][0]));
+      const float c = fmaf(S.q[2], S.m[b][2], fmaf(S.q[1], S.m[b][1], S.q[0] * S.m[b][0]));
+      acc = fmaf(c - a, c + a, acc);
+    }
+    float dE = cAf * acc;
+    if (P.elecA != 0.0f)
+      dE += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
+    const int d = fast_decide(dE, invT, S.u, epsT);
+    if (d >= 0) return d == 1;
+  }
+  float Eo, En;
+  exact_energies<POT>(P, cA, elecE, S, Eo, En);
+  return mcd_accept(En, Eo, T, S.u);
+}
+
+// Proposal of src/evolve.cu:143-163 with a 16-bit axis uniform.
+__device__ __forceinline__ void propose(const float o[3], float av, uint32_t w_angle, uint32_t axis16, float q[3]) {
+  const float va = (2.0f * mcd_u01(w_angle) - 1.0f) * av;
+  const float h = 0.5f * va;  // revolutions: sin(pi*va) = sin(2*pi*h)
+  const float sn = __builtin_amdgcn_sinf(h), cs = __builtin_amdgcn_cosf(h);
+  float q0, q1, q2;
+  if (axis16 < 21824u) {  // u < 0.333
+    q0 = o[0];
+    q1 = o[1] * cs + o[2] * sn;
+    q2 = o[2] * cs - o[1] * sn;
+  } else if (axis16 < 43647u) {  // u < 0.666
+    q0 = o[0] * cs - o[2] * sn;
+    q1 = o[1];
+    q2 = o[2] * cs + o[0] * sn;
+  } else {
+    q0 = o[0] * cs + o[1] * sn;
+    q1 = o[1] * cs - o[0] * sn;
+    q2 = o[2];
+  }
+  const float inv = __builtin_amdgcn_rsqf(q0 * q0 + q1 * q1 + q2 * q2);
+  q[0] = q0 * inv;
+  q[1] = q1 * inv;
+  q[2] = q2 * inv;
+}
+
+// Per-pass constants of the update.
+struct PassConst {
+  float T, elecE, av;
+  uint32_t sweep;
+  double cA;
+  float cAf, eps;
+  float invT, epsT;  // 1/T and eps/T for the FP32 screen
+};
+
+__device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDevState *st) {
+  PassConst k;
+  k.T = st->T;
+  k.elecE = st->elecE;
+  k.av = st->ang_var;
+  k.sweep = st->sweep;
+  k.cA = (double)(-P.A) * 1.5;
+  k.cAf = (float)k.cA;
+  const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
+  k.eps = 6.1035156e-5f * (fabsf(k.cAf) + fabsf(P.elecA) * k.elecE * k.elecE * e2);  // 2^-14 * (...)
+  k.invT = 1.0f / k.T;
+  k.epsT = k.eps / k.T;
+  return k;
+}
+
+// The neighbours of the pair: yp..zm hold, per component, the +y/-y/+z/-z
+// neighbour of site 0 in .x and of site 1 in .y; ov is the x neighbour outside
+// the segment (x = 4q-1 for PAR 0, x = 4q+4 for PAR 1).
+struct PairNbrs {
+  float2 yp[3], ym[3], zp[3], zm[3];
+  float ov[3];
+};
+
+// Proposes and decides both sites of the pair whose first site has linear
+// index sA; c is the segment.  Fills S (for recording) and a.
+template <int POT, int PAR>
+__device__ __forceinline__ void pair_update(const MCDevParams &P, const PassConst &k, const float4 c[3],
+                                            const PairNbrs &nb, uint32_t sA, Site S[2], bool a[2], MCU4 &r) {
+  r = mcd_philox(MCU4{sA, k.sweep, P.stream, 0x4D44u}, P.seed_lo, P.seed_hi);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = PAR + 2 * t;  // element of this site in the segment
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      S[t].o[d] = sel(c[d], e);
+      S[t].m[0][d] = (e == 3) ? nb.ov[d] : sel(c[d], e + 1);  // +x
+      S[t].m[1][d] = (e == 0) ? nb.ov[d] : sel(c[d], e - 1);  // -x
+      S[t].m[2][d] = t ? nb.yp[d].y : nb.yp[d].x;
+      S[t].m[3][d] = t ? nb.ym[d].y : nb.ym[d].x;
+      S[t].m[4][d] = t ? nb.zp[d].y : nb.zp[d].x;
+      S[t].m[5][d] = t ? nb.zm[d].y : nb.zm[d].x;
+    }
+  }
+  const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
+  const uint32_t ax1 = ((r.z & 0xFFu) << 8) | (r.w & 0xFFu);
+  propose(S[0].o, k.av, r.x, ax0, S[0].q);
+  propose(S[1].o, k.av, r.z, ax1, S[1].q);
+  S[0].u = mcd_u01(r.y);
+  S[1].u = mcd_u01(r.w);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) a[t] = decide<POT>(P, k.cA, k.cAf, k.epsT, k.T, k.invT, k.elecE, S[t]);
+}
+
+// Writes the accepted proposals into the segment.
+template <int PAR>
+__device__ __forceinline__ void pair_apply(float4 c[3], const Site S[2], const bool a[2]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (a[0]) {
+      if (PAR == 0) c[d].x = S[0].q[d]; else c[d].y = S[0].q[d];
+    }
+    if (a[1]) {
+      if (PAR == 0) c[d].z = S[1].q[d]; else c[d].w = S[1].q[d];
+    }
+  }
+}
+
+// Record of both sites of a pair (the parity replay reads it back).
+template <int POT>
+__device__ __forceinline__ void pair_record(const MCDevParams &P, const PassConst &k, const MCRecord &rec, int sA,
+                                            const Site S[2], const bool a[2], const MCU4 &r) {
+  const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
+  const uint32_t ax1 = ((r.z & 0xFFu) << 8) | (r.w & 0xFFu);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int s = sA + 2 * t;
+    float Eo, En;
+    exact_energies<POT>(P, k.cA, k.elecE, S[t], Eo, En);
+    rec.u0[s] = mcd_u01(t ? r.z : r.x);
+    rec.u1[s] = (float)(t ? ax1 : ax0) * (1.0f / 65536.0f);
+    rec.u2[s] = S[t].u;
+    rec.px[s] = S[t].q[0];
+    rec.py[s] = S[t].q[1];
+    rec.pz[s] = S[t].q[2];
+    rec.acc[s] = a[t] ? 1 : 0;
+    rec.eo[s] = Eo;
+    rec.en[s] = En;
+  }
+}
+
+}  // namespace mcsu

@@ -1,0 +1,295 @@
+// Slab kernel: one launch per full checkerboard sweep (both colours) of an
+// all-active periodic lattice with Nx % 4 == 0, streaming the lattice once.
+//
+// Why.  A colour pass must see every site (its own colour for the old value,
+// the other colour for the neighbours), so the two-pass sweep moves the
+// lattice through the chip twice and writes it at least once.  Here a sweep
+// reads the directors from one set of planes (`in`) and writes the swept
+// lattice to a second set (`out`); the caller swaps the two afterwards.
+//
+// Tiles.  A workgroup owns TY full rows (all of x) of CZ consecutive planes:
+// rows y0..y0+TY-1, planes z0..z0+CZ-1.  Its threads own one 16-byte x
+// segment [4q, 4q+4) of one of R = TY+4 rows (y0-2 .. y0+TY+1, periodic) and
+// walk z together:
+//   iteration p (z0 <= p < z0+CZ):
+//     red update of plane p+1 on rows y0-1 .. y0+TY   (needs old black sites)
+//     black update of plane p on rows y0 .. y0+TY-1   (needs the new red sites)
+//     store plane p of the owned rows.
+// The red updates of rows y0-1, y0+TY and of planes z0-1, z0+CZ belong to
+// neighbouring tiles; this tile recomputes them.  A proposal and its accept
+// uniform come from a counter-based generator keyed by (site, sweep), and T
+// and ang_var are fixed during a sweep, so every recomputation equals the
+// owner's.  Only owned sites are counted and stored.
+//
+// Data.  Each thread keeps its own column of planes p-1 .. p+3 in registers
+// (z neighbours).  In-plane neighbours come from LDS half rows: the black
+// sites of a row of plane p (old values) and the red sites (new values),
+// each X/2 floats per component, double buffered by plane parity, four half
+// slots in all:
+//   Bbuf(p & 1): black sites of plane p before the sweep (for red updates)
+//   Rbuf(p & 1): red sites of plane p after the red half (for black updates)
+// Within iteration p the kernel reads Bbuf((p+1)&1) and Rbuf(p&1) and writes
+// Rbuf((p+1)&1) and Bbuf(p&1) (black sites of plane p+2); red updates read
+// black sites only and black updates red sites only, so one barrier per
+// iteration orders everything.
+//
+// Parity with the two-pass kernels.  Both colours use the pair update of
+// mc_site_update.h with the same counters as the segment kernel, so a sweep
+// here leaves the lattice, the acceptance count and ang_var exactly as the
+// segment kernel's two passes do.
+#include "mc_kernels.h"
+#include "mc_site_update.h"
+
+namespace {
+
+constexpr int kMaxThreads = 768;
+
+using mcsu::sel;
+
+struct SlabGeom {
+  int X, Y, Z;   // lattice
+  int S;         // segments per row (X / 4)
+  int R;         // rows held (TY + 4)
+  int X2;        // half-row length (X / 2)
+  int TY, CZ;
+};
+
+// Plane index z in [-2, Z+2) wrapped onto the periodic lattice.
+__device__ __forceinline__ int wrap_z(const SlabGeom &g, int z) { return z < 0 ? z + g.Z : (z >= g.Z ? z - g.Z : z); }
+
+__device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], const SlabGeom &g, int z, int y, int q,
+                                         float4 v[3]) {
+  const int zz = wrap_z(g, z);
+  const long long off = (long long)g.X * (y + (long long)g.Y * zz) + 4 * q;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] = *reinterpret_cast<const float4 *>(in[d] + off);
+}
+
+// Half-row address: component d, local row r, half index m.
+__device__ __forceinline__ float *half_at(float *slot, const SlabGeom &g, int d, int r, int m) {
+  return slot + ((size_t)d * g.R + r) * g.X2 + m;
+}
+
+// Stores the two sites of parity par (elements par, par+2) of the segment
+// into a half row.
+// (Bitwise select: a plain ?: on the elements is turned into an indexed load
+// from a scratch copy of the segment.)
+__device__ __forceinline__ float pick(float a, float b, uint32_t mask) {
+  return __uint_as_float((__float_as_uint(a) & ~mask) | (__float_as_uint(b) & mask));
+}
+
+__device__ __forceinline__ void put_half(float *slot, const SlabGeom &g, int r, int q, int par, const float4 v[3]) {
+  const uint32_t mask = 0u - (uint32_t)(par & 1);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float2 w = make_float2(pick(v[d].x, v[d].y, mask), pick(v[d].z, v[d].w, mask));
+    *reinterpret_cast<float2 *>(half_at(slot, g, d, r, 2 * q)) = w;
+  }
+}
+
+// Updates the pair of colour sites of segment q in local row r of plane z.
+// own: the segment (updated in place); zm/zp: the segments of planes z-1 and
+// z+1 in this column; oth: the half-row slot holding the other colour of
+// plane z.  Returns the accepted count; records when asked.
+template <int POT, bool RECORD, int PAR>
+__device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
+                                                float4 own[3], const float4 zm[3], const float4 zp[3], float *oth,
+                                                int r, int q, int y, int z, bool counted, const MCRecord &rec) {
+  mcsu::PairNbrs nb;
+  const int mo = PAR == 0 ? (q == 0 ? g.X2 - 1 : 2 * q - 1) : (q == g.S - 1 ? 0 : 2 * q + 2);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    nb.yp[d] = *reinterpret_cast<const float2 *>(half_at(oth, g, d, r + 1, 2 * q));
+    nb.ym[d] = *reinterpret_cast<const float2 *>(half_at(oth, g, d, r - 1, 2 * q));
+    nb.zp[d] = make_float2(sel(zp[d], PAR), sel(zp[d], PAR + 2));
+    nb.zm[d] = make_float2(sel(zm[d], PAR), sel(zm[d], PAR + 2));
+    nb.ov[d] = *half_at(oth, g, d, r, mo);
+  }
+  const int zz = wrap_z(g, z);
+  const int sA = g.X * (y + g.Y * zz) + 4 * q + PAR;
+  mcsu::Site S[2];
+  bool a[2];
+  MCU4 rr;
+  mcsu::pair_update<POT, PAR>(P, kc, own, nb, (uint32_t)sA, S, a, rr);
+  mcsu::pair_apply<PAR>(own, S, a);
+  if (RECORD && counted) mcsu::pair_record<POT>(P, kc, rec, sA, S, a, rr);
+  return counted ? (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u) : 0u;
+}
+
+template <int POT, bool RECORD>
+__device__ __forceinline__ uint32_t update(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
+                                           int colour, float4 own[3], const float4 zm[3], const float4 zp[3],
+                                           float *oth, int r, int q, int y, int z, bool counted,
+                                           const MCRecord &rec) {
+  if (((y + z + colour) & 1) == 0)
+    return update_pair<POT, RECORD, 0>(P, kc, g, own, zm, zp, oth, r, q, y, z, counted, rec);
+  return update_pair<POT, RECORD, 1>(P, kc, g, own, zm, zp, oth, r, q, y, z, counted, rec);
+}
+
+__device__ __forceinline__ void copy3(float4 dst[3], const float4 src[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dst[d] = src[d];
+}
+
+// Sum over a workgroup of up to 1024 threads; thread 0 gets the total.
+__device__ __forceinline__ uint32_t slab_block_sum(uint32_t v) {
+  __shared__ uint32_t wsum[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += wsum[w];
+  return t;
+}
+
+template <int POT, bool RECORD>
+__global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const float *__restrict__ inx,
+                                                            const float *__restrict__ iny,
+                                                            const float *__restrict__ inz, float *__restrict__ outx,
+                                                            float *__restrict__ outy, float *__restrict__ outz,
+                                                            MCDevState *st, int TY, int CZ, int xcd, MCRecord rec) {
+  extern __shared__ float lds[];
+  SlabGeom g;
+  g.X = P.nx;
+  g.Y = P.ny;
+  g.Z = P.nz;
+  g.S = P.nx >> 2;
+  g.TY = TY;
+  g.CZ = CZ;
+  g.R = TY + 4;
+  g.X2 = P.nx >> 1;
+  const float *in[3] = {inx, iny, inz};
+  float *out[3] = {outx, outy, outz};
+  const mcsu::PassConst kc = mcsu::pass_const(P, st);
+
+  const int nyt = g.Y / TY, nzt = g.Z / CZ, ntiles = nyt * nzt;
+  int tile = blockIdx.x;
+  if (xcd && (ntiles & 7) == 0) tile = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
+  const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
+
+  const int q = threadIdx.x % g.S, r = threadIdx.x / g.S;
+  const bool active = r < g.R;  // threads beyond S*R only join barriers
+  const int y = ((y0 - 2 + r) % g.Y + g.Y) % g.Y;
+  const bool red_row = r >= 1 && r <= g.R - 2;
+  const bool owned_row = r >= 2 && r <= g.R - 3;
+
+  const size_t H = (size_t)3 * g.R * g.X2;  // floats per half slot
+  // Bbuf(par) = lds + par*H, Rbuf(par) = lds + (2+par)*H (computed, not
+  // indexed from an array, which would live in scratch memory).
+  auto Bbuf = [&](int par) { return lds + (size_t)(par & 1) * H; };
+  auto Rbuf = [&](int par) { return lds + (size_t)(2 + (par & 1)) * H; };
+
+  float4 P0[3], P1[3], P2[3], P3[3], Q[3];
+  float4 P4[3] = {};
+  uint32_t acc = 0;
+
+  // Prologue: planes z0-2 .. z0+2; red updates of z0-1 and z0.
+  if (active) {
+    load_seg(in, g, z0 - 2, y, q, Q);
+    load_seg(in, g, z0 - 1, y, q, P0);
+    load_seg(in, g, z0, y, q, P1);
+    load_seg(in, g, z0 + 1, y, q, P2);
+    load_seg(in, g, z0 + 2, y, q, P3);
+    put_half(Bbuf((z0 - 1) & 1), g, r, q, ((y + z0 - 1 + 1) & 1), P0);
+    put_half(Bbuf(z0 & 1), g, r, q, ((y + z0 + 1) & 1), P1);
+  }
+  __syncthreads();
+  if (active && red_row) {
+    update<POT, false>(P, kc, g, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), r, q, y, z0 - 1, false, rec);
+    acc += update<POT, RECORD>(P, kc, g, 0, P1, P0, P2, Bbuf(z0 & 1), r, q, y, z0, owned_row, rec);
+    put_half(Rbuf(z0 & 1), g, r, q, ((y + z0) & 1), P1);
+  }
+  __syncthreads();
+  if (active) put_half(Bbuf((z0 + 1) & 1), g, r, q, ((y + z0 + 1 + 1) & 1), P2);
+  __syncthreads();
+
+  for (int it = 0; it < CZ; ++it) {
+    const int p = z0 + it;
+    if (active) {
+      if (it <= CZ - 2) load_seg(in, g, p + 3, y, q, P4);
+      if (it <= CZ - 2) put_half(Bbuf(p & 1), g, r, q, ((y + p + 2 + 1) & 1), P3);
+      if (red_row) {
+        acc += update<POT, RECORD>(P, kc, g, 0, P2, P1, P3, Bbuf((p + 1) & 1), r, q, y, p + 1,
+                                   owned_row && it <= CZ - 2, rec);
+        if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), g, r, q, ((y + p + 1) & 1), P2);
+      }
+      if (owned_row) {
+        acc += update<POT, RECORD>(P, kc, g, 1, P1, P0, P2, Rbuf(p & 1), r, q, y, p, true, rec);
+        const long long off = (long long)g.X * (y + (long long)g.Y * p) + 4 * q;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(out[d] + off) = P1[d];
+      }
+    }
+    __syncthreads();
+    copy3(P0, P1);
+    copy3(P1, P2);
+    copy3(P2, P3);
+    copy3(P3, P4);
+  }
+  const uint32_t bsum = slab_block_sum(acc);
+  mcd_finish_sweep(P, st, bsum, true);
+}
+
+template <int POT>
+void launch_t(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabShape &s, const MCRecord *rec) {
+  const int threads = (L.P.nx / 4) * (s.ty + 4);
+  const int grid = (L.P.ny / s.ty) * (L.P.nz / s.cz);
+  const size_t lds = mc_slab_lds_bytes(L.P, s.ty);
+  if (rec)
+    k_sweep_slab<POT, true><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, s.cz,
+                                                              s.xcd, *rec);
+  else
+    k_sweep_slab<POT, false><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, s.cz,
+                                                               s.xcd, MCRecord{});
+}
+
+}  // namespace
+
+size_t mc_slab_lds_bytes(const MCDevParams &P, int ty) { return (size_t)4 * 3 * (ty + 4) * (P.nx / 2) * sizeof(float); }
+
+bool mc_slab_shape_ok(const MCDevParams &P, const MCSlabShape &s, size_t lds_limit) {
+  if (s.ty < 1 || s.cz < 1 || s.ty > P.ny || s.cz > P.nz) return false;
+  if (P.ny % s.ty || P.nz % s.cz) return false;
+  if ((P.nx / 4) * (s.ty + 4) > kMaxThreads) return false;
+  return mc_slab_lds_bytes(P, s.ty) + 256 <= lds_limit;  // + the static reduction array
+}
+
+int mc_slab_prepare(size_t lds_bytes) {
+  // Dynamic LDS beyond 64 KiB: ask for it on every instantiation.  The HIP
+  // runtime may reject the attribute on devices whose per-workgroup limit is
+  // already the full LDS; that is not an error (a launch beyond the limit
+  // still fails cleanly and is reported by the caller).
+  const void *fns[] = {
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true>),
+  };
+  int failed = 0;
+  for (const void *f : fns)
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      ++failed;
+    }
+  return failed;
+}
+
+void mc_launch_sweep_slab(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabShape &s,
+                          const MCRecord *rec) {
+  switch (L.P.potential) {
+    case MCD_GHRL:
+      launch_t<MCD_GHRL>(L, ox, oy, oz, s, rec);
+      break;
+    case MCD_PEAR:
+      launch_t<MCD_PEAR>(L, ox, oy, oz, s, rec);
+      break;
+    default:
+      launch_t<MCD_LL>(L, ox, oy, oz, s, rec);
+      break;
+  }
+}

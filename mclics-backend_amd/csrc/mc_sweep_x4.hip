@@ -10,119 +10,21 @@
 // accepted.  Neighbouring threads read neighbouring segments, so every load
 // of a wave is a contiguous 1 KiB run per plane.  Black sites written back
 // during a red pass carry the values just read; no other thread writes them
-// in that pass, so the in-place update is race free.
-//
-// Randomness.  One Philox4x32-10 block per thread and pass (counter = (index
-// of the first site, sweep, stream, 0x4D44)) feeds both sites: words 0/2 give
-// the angle uniforms, words 1/3 the accept uniforms (top 24 bits each) and the
-// low bytes of the four words the two 16-bit axis uniforms.
-//
-// Accept decision.  For Lebwohl-Lasher the energy difference is first formed
-// in FP32 with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2), whose distance from the
-// reference's own FP32/FP64 arithmetic is bounded by 2^-14*(1.5|A| + |field|)
-// (six bonds, a few roundings each, with a 4x margin).  A hardware exp2 then
-// decides every move whose uniform lies outside that error band around
-// exp(-dE/T); moves inside the band (about 1e-4 of them at T ~ 1) are decided
-// with the exact reference arithmetic, so every decision equals the one the
-// CPU path makes.  GHRL and pear always use the exact arithmetic.
+// in that pass, so the in-place update is race free.  Proposal, randomness
+// and accept rule are those of mc_site_update.h.
 #include "mc_kernels.h"
+#include "mc_site_update.h"
 
 namespace {
 
 constexpr int kBlock = MCD_BLOCK;
-
-__device__ __forceinline__ float sel(const float4 &v, int e) {
-  return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
-}
-
-// 1 accept, 0 reject, -1 inside the error band (decide exactly).
-__device__ __forceinline__ int fast_decide(float dE, float T, float u, float eps) {
-  if (u == 0.0f) return -1;
-  const float x = -dE / T;
-  const float rel = eps / T + fabsf(x) * 4.76837158e-7f + 1.9073486e-6f;  // + |x| 2^-21 + 2^-19
-  if (!(rel < 0.125f)) return -1;
-  const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
-  if (fabsf(u - p) <= p * rel) return -1;
-  return u < p ? 1 : 0;
-}
-
-struct Site {
-  float o[3];     // current director
-  float q[3];     // proposal
-  float m[6][3];  // neighbours +x,-x,+y,-y,+z,-z
-  float u;        // accept uniform
-};
-
-template <int POT>
-__device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, float elecE, const Site &S,
-                                               float &Eo, float &En) {
-  double so = 0.0, sn = 0.0;
-#pragma unroll
-  for (int b = 0; b < 6; ++b) {
-    so += (double)mcd_bond<POT>(P, cA, b, S.o[0], S.o[1], S.o[2], S.m[b][0], S.m[b][1], S.m[b][2]);
-    sn += (double)mcd_bond<POT>(P, cA, b, S.q[0], S.q[1], S.q[2], S.m[b][0], S.m[b][1], S.m[b][2]);
-  }
-  Eo = (float)so;
-  En = (float)sn;
-  if (P.elecA != 0.0f) {
-    Eo += mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
-    En += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]);
-  }
-}
-
-template <int POT>
-__device__ __forceinline__ bool decide(const MCDevParams &P, double cA, float cAf, float eps, float T, float elecE,
-                                       const Site &S) {
-  if (POT == MCD_LL) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-      const float a = fmaf(S.o[2], S.m[b][2], fmaf(S.o[1], S.m[b][1], S.o[0] * S.m[b][0]));
-      const float c = fmaf(S.q[2], S.m[b][2], fmaf(S.q[1], S.m[b][1], S.q[0] * S.m[b][0]));
-      acc = fmaf(c - a, c + a, acc);
-    }
-    float dE = cAf * acc;
-    if (P.elecA != 0.0f)
-      dE += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
-    const int d = fast_decide(dE, T, S.u, eps);
-    if (d >= 0) return d == 1;
-  }
-  float Eo, En;
-  exact_energies<POT>(P, cA, elecE, S, Eo, En);
-  return mcd_accept(En, Eo, T, S.u);
-}
-
-// Proposal of src/evolve.cu:143-163 with a 16-bit axis uniform.
-__device__ __forceinline__ void propose(const float o[3], float av, uint32_t w_angle, uint32_t axis16, float q[3]) {
-  const float va = (2.0f * mcd_u01(w_angle) - 1.0f) * av;
-  const float h = 0.5f * va;  // revolutions: sin(pi*va) = sin(2*pi*h)
-  const float sn = __builtin_amdgcn_sinf(h), cs = __builtin_amdgcn_cosf(h);
-  float q0, q1, q2;
-  if (axis16 < 21824u) {  // u < 0.333
-    q0 = o[0];
-    q1 = o[1] * cs + o[2] * sn;
-    q2 = o[2] * cs - o[1] * sn;
-  } else if (axis16 < 43647u) {  // u < 0.666
-    q0 = o[0] * cs - o[2] * sn;
-    q1 = o[1];
-    q2 = o[2] * cs + o[0] * sn;
-  } else {
-    q0 = o[0] * cs + o[1] * sn;
-    q1 = o[1] * cs - o[0] * sn;
-    q2 = o[2];
-  }
-  const float inv = __builtin_amdgcn_rsqf(q0 * q0 + q1 * q1 + q2 * q2);
-  q[0] = q0 * inv;
-  q[1] = q1 * inv;
-  q[2] = q2 * inv;
-}
+using mcsu::sel;
 
 // One thread's segment update; PAR is the x parity of the first site of the
 // pass colour in this row (sites at 4q+PAR and 4q+PAR+2).
 template <int POT, bool RECORD, int PAR>
-__device__ __forceinline__ uint32_t segment(const MCDevParams &P, float *__restrict__ nx, float *__restrict__ ny,
-                                            float *__restrict__ nz, int q, int j, int k, uint32_t sweep, float T,
-                                            float elecE, float av, double cA, float cAf, float eps,
+__device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::PassConst &kc, float *__restrict__ nx,
+                                            float *__restrict__ ny, float *__restrict__ nz, int q, int j, int k,
                                             const MCRecord &rec) {
   const int X = P.nx, Y = P.ny, Z = P.nz;
   const int nseg = X >> 2;
@@ -135,73 +37,32 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, float *__restr
   const int own = row + x0;
   const int out = PAR == 0 ? row + (q == 0 ? X - 1 : x0 - 1) : row + (q == nseg - 1 ? 0 : x0 + 4);
   float *pl[3] = {nx, ny, nz};
-  float4 c[3], yp[3], ym[3], zp[3], zm[3];
-  float ov[3];
+  float4 c[3];
+  mcsu::PairNbrs nb;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     c[d] = *reinterpret_cast<const float4 *>(pl[d] + own);
-    yp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
-    ym[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
-    zp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
-    zm[d] = *reinterpret_cast<const float4 *>(pl[d] + r_km);
-    ov[d] = pl[d][out];
+    const float4 yp = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
+    const float4 ym = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
+    const float4 zp = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
+    const float4 zm = *reinterpret_cast<const float4 *>(pl[d] + r_km);
+    nb.yp[d] = make_float2(sel(yp, PAR), sel(yp, PAR + 2));
+    nb.ym[d] = make_float2(sel(ym, PAR), sel(ym, PAR + 2));
+    nb.zp[d] = make_float2(sel(zp, PAR), sel(zp, PAR + 2));
+    nb.zm[d] = make_float2(sel(zm, PAR), sel(zm, PAR + 2));
+    nb.ov[d] = pl[d][out];
   }
   const int sA = own + PAR;  // linear index of the first site
-  const MCU4 r = mcd_philox(MCU4{(uint32_t)sA, sweep, P.stream, 0x4D44u}, P.seed_lo, P.seed_hi);
-  Site S[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int e = PAR + 2 * t;  // element of this site in the segment
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      S[t].o[d] = sel(c[d], e);
-      S[t].m[0][d] = (e == 3) ? ov[d] : sel(c[d], e + 1);  // +x
-      S[t].m[1][d] = (e == 0) ? ov[d] : sel(c[d], e - 1);  // -x
-      S[t].m[2][d] = sel(yp[d], e);
-      S[t].m[3][d] = sel(ym[d], e);
-      S[t].m[4][d] = sel(zp[d], e);
-      S[t].m[5][d] = sel(zm[d], e);
-    }
-  }
-  const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
-  const uint32_t ax1 = ((r.z & 0xFFu) << 8) | (r.w & 0xFFu);
-  propose(S[0].o, av, r.x, ax0, S[0].q);
-  propose(S[1].o, av, r.z, ax1, S[1].q);
-  S[0].u = mcd_u01(r.y);
-  S[1].u = mcd_u01(r.w);
+  mcsu::Site S[2];
   bool a[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) a[t] = decide<POT>(P, cA, cAf, eps, T, elecE, S[t]);
+  MCU4 r;
+  mcsu::pair_update<POT, PAR>(P, kc, c, nb, (uint32_t)sA, S, a, r);
   if (a[0] || a[1]) {
+    mcsu::pair_apply<PAR>(c, S, a);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float4 v = c[d];
-      if (a[0]) {
-        if (PAR == 0) v.x = S[0].q[d]; else v.y = S[0].q[d];
-      }
-      if (a[1]) {
-        if (PAR == 0) v.z = S[1].q[d]; else v.w = S[1].q[d];
-      }
-      *reinterpret_cast<float4 *>(pl[d] + own) = v;
-    }
+    for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(pl[d] + own) = c[d];
   }
-  if (RECORD) {
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int s = sA + 2 * t;
-      float Eo, En;
-      exact_energies<POT>(P, cA, elecE, S[t], Eo, En);
-      rec.u0[s] = mcd_u01(t ? r.z : r.x);
-      rec.u1[s] = (float)(t ? ax1 : ax0) * (1.0f / 65536.0f);
-      rec.u2[s] = S[t].u;
-      rec.px[s] = S[t].q[0];
-      rec.py[s] = S[t].q[1];
-      rec.pz[s] = S[t].q[2];
-      rec.acc[s] = a[t] ? 1 : 0;
-      rec.eo[s] = Eo;
-      rec.en[s] = En;
-    }
-  }
+  if (RECORD) mcsu::pair_record<POT>(P, kc, rec, sA, S, a, r);
   return (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u);
 }
 
@@ -209,12 +70,7 @@ template <int POT, bool RECORD>
 __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__restrict__ nx, float *__restrict__ ny,
                                                      float *__restrict__ nz, MCDevState *st, int colour, int finalize,
                                                      int xcd, MCRecord rec) {
-  const float T = st->T, elecE = st->elecE, av = st->ang_var;
-  const uint32_t sweep = st->sweep;
-  const double cA = (double)(-P.A) * 1.5;
-  const float cAf = (float)cA;
-  const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
-  const float eps = 6.1035156e-5f * (fabsf(cAf) + fabsf(P.elecA) * elecE * elecE * e2);  // 2^-14 * (...)
+  const mcsu::PassConst kc = mcsu::pass_const(P, st);
   const int nseg = P.nx >> 2;
   const int total = nseg * P.ny * P.nz;
   uint32_t acc = 0;
@@ -226,9 +82,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
     const int rowi = t / nseg;
     const int j = rowi % P.ny, k = rowi / P.ny;
     if (((j + k + colour) & 1) == 0)
-      acc += segment<POT, RECORD, 0>(P, nx, ny, nz, q, j, k, sweep, T, elecE, av, cA, cAf, eps, rec);
+      acc += segment<POT, RECORD, 0>(P, kc, nx, ny, nz, q, j, k, rec);
     else
-      acc += segment<POT, RECORD, 1>(P, nx, ny, nz, q, j, k, sweep, T, elecE, av, cA, cAf, eps, rec);
+      acc += segment<POT, RECORD, 1>(P, kc, nx, ny, nz, q, j, k, rec);
   }
   const uint32_t bsum = mcd_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, finalize != 0);
@@ -289,12 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
     const int r = tile_rep[tile];
     const MCDevParams P = params[r];
     MCDevState *st = states + r;
-    const float T = st->T, elecE = st->elecE, av = st->ang_var;
-    const uint32_t sweep = st->sweep;
-    const double cA = (double)(-P.A) * 1.5;
-    const float cAf = (float)cA;
-    const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
-    const float eps = 6.1035156e-5f * (fabsf(cAf) + fabsf(P.elecA) * elecE * elecE * e2);
+    const mcsu::PassConst kc = mcsu::pass_const(P, st);
     const long long off = rep_off[r];
     const int nseg = P.nx >> 2;
     const int total = nseg * P.ny * P.nz;
@@ -305,11 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
       const int rowi = t / nseg;
       const int j = rowi % P.ny, k = rowi / P.ny;
       if (((j + k + colour) & 1) == 0)
-        acc = segment<POT, false, 0>(P, nx + off, ny + off, nz + off, q, j, k, sweep, T, elecE, av, cA, cAf, eps,
-                                     MCRecord{});
+        acc = segment<POT, false, 0>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
       else
-        acc = segment<POT, false, 1>(P, nx + off, ny + off, nz + off, q, j, k, sweep, T, elecE, av, cA, cAf, eps,
-                                     MCRecord{});
+        acc = segment<POT, false, 1>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
     }
     const uint32_t bsum = mcd_block_sum(acc);
     if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);

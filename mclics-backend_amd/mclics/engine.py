@@ -20,7 +20,7 @@ MC_MAX_SURF = 8
 FREE, PERIODIC = 0, 1
 LL, GHRL, PEAR = 0, 1, 2
 ORDER_CHECKER, ORDER_REF8 = 0, 1
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_X4 = 0, 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_X4, KERNEL_SLAB = 0, 1, 2, 3
 ANCHORING = {
     "homeotropic": 0,
     "rp": 1,
@@ -58,6 +58,8 @@ EXPORTS = (
     "mc_pass",
     "mc_record_sweep",
     "mc_set_tuning",
+    "mc_set_slab_tile",
+    "mc_get_slab_tile",
     "mc_autotune",
     "mc_batch_create",
     "mc_batch_destroy",
@@ -169,6 +171,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     dp = C.POINTER(C.c_double)
     u32p = C.POINTER(C.c_uint32)
     lib.mc_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.mc_set_slab_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    lib.mc_get_slab_tile.argtypes = [vp, ip, ip, ip]
     lib.mc_autotune.argtypes = [vp, ip, ip, f32p]
     lib.mc_batch_create.argtypes = [C.c_int, C.POINTER(McParams), C.POINTER(f32p), C.POINTER(vp)]
     lib.mc_batch_destroy.argtypes = [vp]
@@ -339,6 +343,15 @@ class Lattice:
 
     def set_tuning(self, grid: int, xcd: int) -> None:
         _check(self._lib.mc_set_tuning(self._h, grid, xcd))
+
+    def set_slab_tile(self, ty: int, cz: int, xcd: int = 1) -> None:
+        """Slab-kernel tile: ty full rows of cz planes per workgroup (0, 0: default)."""
+        _check(self._lib.mc_set_slab_tile(self._h, ty, cz, xcd))
+
+    def slab_tile(self):
+        ty, cz, x = C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.mc_get_slab_tile(self._h, C.byref(ty), C.byref(cz), C.byref(x)))
+        return ty.value, cz.value, x.value
 
     def autotune(self):
         g, x, us = C.c_int(), C.c_int(), C.c_float()

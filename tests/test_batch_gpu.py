@@ -26,7 +26,9 @@ def test_batch_replicas_match_single_lattices():
         acc, sw, av = b.state()
         for r, cfg in enumerate(cfgs):
             with mclics.Lattice(cfg, ics[r]) as lat:
-                assert lat.kernel() == mclics.KERNEL_X4
+                # AUTO picks the slab kernel here; it draws the same numbers and
+                # evolves the lattice exactly as the segment kernel does
+                assert lat.kernel() in (mclics.KERNEL_SLAB, mclics.KERNEL_X4)
                 lat.sweeps(12)
                 d = lat.directors()
                 E1, S1, _ = lat.measure()

@@ -152,11 +152,40 @@ def test_learned_sweep_decisions_are_exact_mh():
     np.testing.assert_array_equal(state.reshape(-1), end)
 
 
+def test_learned_proposals_follow_their_density():
+    # The sampler must draw from the density the acceptance ratio uses:
+    # c = mu.q has density kappa cosh(kappa c) / (2 sinh kappa) on [-1, 1], so
+    # U = (sinh(kappa c) + sinh kappa) / (2 sinh kappa) is uniform (probability
+    # integral transform, each site with its own kappa).
+    from scipy import stats
+    L = 32
+    with mclics.Network(seed=9) as net:
+        lat = equilibrated(L, 1.1, 100, 41)
+        for k in range(2):
+            lat.sweeps(10)
+            net.add_samples(lat, 8192, seed=900 + k)
+        net.train(epochs=3, batch=1024, lr=2e-3)
+        lat.set_learned(net)
+        rec = lat.record_learned_sweep()
+        lat.close()
+    mu = rec["mu"].reshape(-1, 3).astype(np.float64)
+    q = rec["prop"].reshape(-1, 3).astype(np.float64)
+    kap = np.exp(rec["logk"].astype(np.float64))
+    c = np.clip(np.sum(mu * q, axis=1), -1.0, 1.0)
+    # (sinh(k c) + sinh k) / (2 sinh k), written stably for large kappa
+    U = 0.5 * (np.exp(kap * (c - 1)) - np.exp(-kap * (c + 1))) / (1 - np.exp(-2 * kap)) + 0.5
+    assert kap.min() > 0 and np.isfinite(U).all()
+    res = stats.kstest(U, "uniform")
+    assert res.pvalue > 1e-3, (res, np.percentile(kap, [5, 50, 95]))
+    # azimuth and sign: the projection on a fixed axis orthogonal to mu has mean 0
+    assert abs(np.mean(np.sign(np.sum(mu * q, axis=1)))) < 4 / math.sqrt(len(c))
+
+
 @pytest.mark.slow
 def test_learned_and_plain_agree_within_3_sigma():
     # Same lattice, temperatures on both sides of the transition; R replicas
     # per method: |mean_A - mean_B| <= 3 sqrt(se_A^2 + se_B^2).
-    L, R, nequil, nmeas = 8, 8, 400, 1600
+    L, R, nequil, nmeas = 8, 16, 400, 1600
     with mclics.Network(seed=6) as net:
         lat = equilibrated(16, 1.1, 300, 31)
         for k in range(3):

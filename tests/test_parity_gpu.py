@@ -256,13 +256,15 @@ def test_create_rejects_bad_input():
             lat.pass_inject(11, n, np.zeros(64, np.float32))
 
 
+@pytest.mark.parametrize("kernel", ["x4", "slab"])
 @pytest.mark.parametrize("pot,T,field", [(O.LL, 1.13, False), (O.LL, 0.6, True), (O.LL, 1e-4, False),
                                          (O.LL, 40.0, False), (O.GHRL, 0.5, True), (O.PEAR, 0.8, False)])
-def test_segment_kernel_replay(pot, T, field):
-    # The x4 segment kernel (fast FP32 screen + exact fallback for LL) must
-    # make exactly the CPU path's decisions: replay recorded sweeps in the
-    # oracle.  T = 1e-4 sends nearly every move through the exact fallback,
-    # T = 40 makes most of them fast accepts.
+def test_segment_kernel_replay(pot, T, field, kernel):
+    # The x4 segment kernel and the slab kernel (fast FP32 screen + exact
+    # fallback for LL) must make exactly the CPU path's decisions: replay
+    # recorded sweeps in the oracle, colour 0 then colour 1.  T = 1e-4 sends
+    # nearly every move through the exact fallback, T = 40 makes most of them
+    # fast accepts.
     rng = np.random.default_rng(15)
     nx, ny, nz = 16, 8, 12
     nt = nx * ny * nz
@@ -280,9 +282,12 @@ def test_segment_kernel_replay(pot, T, field):
     n = unit(rng, nt)
     ref = n.copy()
     pt = np.ones(nt, np.int32)
+    kind = mclics.KERNEL_X4 if kernel == "x4" else mclics.KERNEL_SLAB
     with mclics.Lattice(cfg, n) as lat:
-        lat.set_kernel(mclics.KERNEL_X4)
-        assert lat.kernel() == mclics.KERNEL_X4
+        lat.set_kernel(kind)
+        assert lat.kernel() == kind
+        if kernel == "slab":
+            lat.set_slab_tile(4, 3)  # 2 row tiles x 4 plane tiles: halos cross tiles on y and z
         for w in range(3):
             av = lat.ang_var()
             pre = ref.copy()
@@ -321,10 +326,106 @@ def test_segment_kernel_selection_rules():
         assert lat.kernel() == mclics.KERNEL_GENERIC
         with pytest.raises(mclics.EngineError):
             lat.set_kernel(mclics.KERNEL_X4)
+        with pytest.raises(mclics.EngineError):
+            lat.set_kernel(mclics.KERNEL_SLAB)
     n = unit(rng, 8 * 4 * 4)
     with mclics.Lattice(mclics.Config(8, 4, 4), n) as lat:
+        assert lat.kernel() == mclics.KERNEL_SLAB
+        lat.set_kernel(mclics.KERNEL_X4)
         assert lat.kernel() == mclics.KERNEL_X4
         lat.set_kernel(mclics.KERNEL_GENERIC)
         assert lat.kernel() == mclics.KERNEL_GENERIC
+        lat.set_kernel(mclics.KERNEL_AUTO)
+        assert lat.kernel() == mclics.KERNEL_SLAB
+        for bad in ((3, 4), (4, 3), (0, 4), (64, 4)):
+            with pytest.raises(mclics.EngineError):
+                lat.set_slab_tile(*bad)
     with mclics.Lattice(mclics.Config(8, 4, 4, order=mclics.ORDER_REF8), n) as lat:
         assert lat.kernel() == mclics.KERNEL_GENERIC
+
+
+def twin_run(cfg, n, sweeps, shape=None, graph=False):
+    """(directors, acceptance, ang_var, E) after `sweeps` sweeps with the
+    segment kernel and with the slab kernel (tile shape `shape`)."""
+    out = []
+    for kind in (mclics.KERNEL_X4, mclics.KERNEL_SLAB):
+        with mclics.Lattice(cfg, n) as lat:
+            lat.set_kernel(kind)
+            if kind == mclics.KERNEL_SLAB and shape is not None:
+                lat.set_slab_tile(*shape)
+            lat.sweeps(sweeps, graph=graph)
+            lat.sync()
+            out.append((lat.directors(), lat.acceptance(), lat.ang_var(), lat.measure()[0]))
+    return out
+
+
+@pytest.mark.parametrize("dims,shape", [((16, 8, 12), (8, 12, 0)), ((16, 8, 12), (2, 2, 1)),
+                                        ((16, 8, 12), (1, 4, 0)), ((8, 2, 2), (2, 2, 1)),
+                                        ((4, 6, 10), (6, 5, 0)), ((32, 16, 16), (4, 4, 1)),
+                                        ((12, 10, 8), (10, 8, 1)), ((64, 16, 32), None)])
+def test_slab_matches_segment_kernel(dims, shape):
+    # The slab kernel recomputes the red updates of its halo rows and planes;
+    # with any tile shape (one tile, tiles of one row or plane pair, tiles
+    # that wrap onto themselves) the chain must equal the segment kernel's
+    # bit for bit: state, acceptance counts, ang_var and energy.
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * 10000 + ny * 100 + nz)
+    n = unit(rng, nx * ny * nz)
+    for T in (0.7, 1.15):
+        cfg = mclics.Config(nx, ny, nz, T=T, seed=40 + nx)
+        a, b = twin_run(cfg, n, 7, shape)
+        assert np.array_equal(a[0], b[0]), (dims, shape, T)
+        assert a[1:] == b[1:]
+
+
+def test_slab_matches_segment_kernel_ghrl_field():
+    nx, ny, nz = 16, 12, 8
+    n = unit(np.random.default_rng(17), nx * ny * nz)
+    po = O.params(nx, ny, nz, potential=O.GHRL, k=GHRL_K, elec=(0.2, 0.1, 1.0), elecA=0.9, elecE=0.8)
+    cfg = mclics.Config(nx, ny, nz, potential=O.GHRL, ghrl=O.ghrl_tuple(po), elec=(0.2, 0.1, 1.0), elecA=0.9,
+                        elecE=0.8, T=0.5, seed=3)
+    a, b = twin_run(cfg, n, 5, (4, 2, 1))
+    assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+
+
+def test_slab_full_size_256_matches_segment_kernel():
+    # The headline lattice with the default (device-chosen) tile, an odd
+    # number of sweeps through a graph (the plane sets end swapped).
+    L = 256
+    n = mclics.random_ic(L, L, L, seed=2)
+    a, b = twin_run(mclics.Config(L, L, L, T=1.13, seed=2), n, 3, graph=True)
+    assert a[1:] == b[1:]
+    assert np.array_equal(a[0], b[0])
+
+
+def test_slab_autotune_leaves_chain_and_keeps_parity():
+    nx, ny, nz = 32, 16, 16
+    n = unit(np.random.default_rng(18), nx * ny * nz)
+    cfg = mclics.Config(nx, ny, nz, T=0.9, seed=8)
+    with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
+        assert a.kernel() == mclics.KERNEL_SLAB
+        a.sweeps(3)
+        b.sweeps(3)
+        grid, xcd, us = a.autotune()
+        ty, cz, x = a.slab_tile()
+        assert grid == (ny // ty) * (nz // cz) and x == xcd and us > 0
+        a.sweeps(4, graph=True)
+        b.sweeps(4)
+        a.sync()
+        b.sync()
+        assert np.array_equal(a.directors(), b.directors())
+        assert a.acceptance() == b.acceptance() and a.ang_var() == b.ang_var()
+
+
+def test_slab_measure_series_reads_the_current_planes():
+    # mc_measure_series interleaves sweeps and measurements; with the slab
+    # kernel the lattice alternates between two plane sets.
+    nx, ny, nz = 16, 8, 8
+    n = unit(np.random.default_rng(19), nx * ny * nz)
+    cfg = mclics.Config(nx, ny, nz, T=0.8, seed=9)
+    with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
+        E, S = a.measure_series(5)
+        for w in range(5):
+            b.sweeps(1)
+            e, s_, _ = b.measure()
+            assert E[w] == e and S[w] == s_, w
