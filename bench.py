@@ -120,8 +120,11 @@ def learned_leg(n_meas=30000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14
     return out
 
 
-def load_traffic():
-    path = os.path.join(REPO, "profiles", "sweep_pass_ll256_traffic.json")
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` at 256^3 from the committed byte-counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the copy
+    calibration), or None."""
+    path = os.path.join(REPO, "profiles", f"{kernel}_ll256_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -150,7 +153,7 @@ def copy_probe(nbytes=201326592, reps=20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
@@ -231,7 +234,7 @@ def main():
                                "T in {1.2, 1.13, 1.0, 0.6}", "lattice": [L, L, L], "temperatures": list(TEMPS),
                    "sweeps_per_T": per_t, "order": "checkerboard (2 passes/sweep)", "seed": 2},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(kname.split()[0]),
                      "kernel": kname, "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_pass_s * 1e6, "launches_per_sweep": launches_per_sweep,
                      "slab_tile": tile},
