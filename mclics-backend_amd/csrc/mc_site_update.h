@@ -70,47 +70,26 @@ __device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, 
   }
 }
 
-template <int POT>
-__device__ __forceinline__ bool decide(const MCDevParams &P, double cA, float cAf, float epsT, float T, float invT,
-                                       float elecE, const Site &S) {
-  if (POT == MCD_LL) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-      const float a = fmaf(S.o[2], S.m[b][2], fmaf(S.o[1], S.m[b][1], S.o[0] * S.m[b][0]));
-      const float c = fmaf(S.q[2], S.m[b][2], fmaf(S.q[1], S.m[b][1], S.q[0] * S.m[b][0]));
-      acc = fmaf(c - a, c + a, acc);
-    }
-    float dE = cAf * acc;
-    if (P.elecA != 0.0f)
-      dE += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
-    const int d = fast_decide(dE, invT, S.u, epsT);
-    if (d >= 0) return d == 1;
-  }
-  float Eo, En;
-  exact_energies<POT>(P, cA, elecE, S, Eo, En);
-  return mcd_accept(En, Eo, T, S.u);
-}
-
 // Proposal of src/evolve.cu:143-163 with a 16-bit axis uniform.
 __device__ __forceinline__ void propose(const float o[3], float av, uint32_t w_angle, uint32_t axis16, float q[3]) {
   const float va = (2.0f * mcd_u01(w_angle) - 1.0f) * av;
   const float h = 0.5f * va;  // revolutions: sin(pi*va) = sin(2*pi*h)
   const float sn = __builtin_amdgcn_sinf(h), cs = __builtin_amdgcn_cosf(h);
-  float q0, q1, q2;
-  if (axis16 < 21824u) {  // u < 0.333
-    q0 = o[0];
-    q1 = o[1] * cs + o[2] * sn;
-    q2 = o[2] * cs - o[1] * sn;
-  } else if (axis16 < 43647u) {  // u < 0.666
-    q0 = o[0] * cs - o[2] * sn;
-    q1 = o[1];
-    q2 = o[2] * cs + o[0] * sn;
-  } else {
-    q0 = o[0] * cs + o[1] * sn;
-    q1 = o[1] * cs - o[0] * sn;
-    q2 = o[2];
-  }
+  // Rotation about x (u < 0.333), y (u < 0.666) or z, as one rotation of the
+  // pair (a, b) = (o1, o2), (o2, o0) or (o0, o1):
+  //   x: q1 = o1 cs + o2 sn, q2 = o2 cs - o1 sn
+  //   y: q2 = o2 cs + o0 sn, q0 = o0 cs - o2 sn
+  //   z: q0 = o0 cs + o1 sn, q1 = o1 cs - o0 sn
+  // Selects instead of branches (the axis differs from lane to lane); the
+  // products and sums are the same operations in the same order.
+  const bool ax = axis16 < 21824u, ay = !ax && axis16 < 43647u;
+  const float a = ax ? o[1] : (ay ? o[2] : o[0]);
+  const float b = ax ? o[2] : (ay ? o[0] : o[1]);
+  const float an = a * cs + b * sn;
+  const float bn = b * cs - a * sn;
+  const float q0 = ax ? o[0] : (ay ? bn : an);
+  const float q1 = ax ? an : (ay ? o[1] : bn);
+  const float q2 = ax ? bn : (ay ? an : o[2]);
   const float inv = __builtin_amdgcn_rsqf(q0 * q0 + q1 * q1 + q2 * q2);
   q[0] = q0 * inv;
   q[1] = q1 * inv;
@@ -139,6 +118,22 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
   k.invT = 1.0f / k.T;
   k.epsT = k.eps / k.T;
   return k;
+}
+
+// FP32 screen of one Lebwohl-Lasher move: 1 accept, 0 reject, -1 decide
+// exactly.
+__device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const float a = fmaf(S.o[2], S.m[b][2], fmaf(S.o[1], S.m[b][1], S.o[0] * S.m[b][0]));
+    const float c = fmaf(S.q[2], S.m[b][2], fmaf(S.q[1], S.m[b][1], S.q[0] * S.m[b][0]));
+    acc = fmaf(c - a, c + a, acc);
+  }
+  float dE = k.cAf * acc;
+  if (P.elecA != 0.0f)
+    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
+  return fast_decide(dE, k.invT, S.u, k.epsT);
 }
 
 // The neighbours of the pair: yp..zm hold, per component, the +y/-y/+z/-z
@@ -175,8 +170,31 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   propose(S[1].o, k.av, r.z, ax1, S[1].q);
   S[0].u = mcd_u01(r.y);
   S[1].u = mcd_u01(r.w);
+  if (POT == MCD_LL) {
+    // Screen both sites, then one shared (rarely executed) exact evaluation
+    // for whichever sites fell inside the error band.
+    const int d0 = screen_ll(P, k, S[0]), d1 = screen_ll(P, k, S[1]);
+    a[0] = d0 == 1;
+    a[1] = d1 == 1;
+    if (d0 < 0 || d1 < 0) {
+#pragma unroll 1
+      for (int t = 0; t < 2; ++t) {
+        if ((t ? d1 : d0) >= 0) continue;
+        const Site &X = t ? S[1] : S[0];
+        float Eo, En;
+        exact_energies<POT>(P, k.cA, k.elecE, X, Eo, En);
+        const bool at = mcd_accept(En, Eo, k.T, X.u);
+        if (t) a[1] = at; else a[0] = at;
+      }
+    }
+    return;
+  }
 #pragma unroll
-  for (int t = 0; t < 2; ++t) a[t] = decide<POT>(P, k.cA, k.cAf, k.epsT, k.T, k.invT, k.elecE, S[t]);
+  for (int t = 0; t < 2; ++t) {  // GHRL and pear: the exact arithmetic throughout
+    float Eo, En;
+    exact_energies<POT>(P, k.cA, k.elecE, S[t], Eo, En);
+    a[t] = mcd_accept(En, Eo, k.T, S[t].u);
+  }
 }
 
 // Writes the accepted proposals into the segment.

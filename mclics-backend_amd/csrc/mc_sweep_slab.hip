@@ -48,26 +48,29 @@ using mcsu::sel;
 
 struct SlabGeom {
   int X, Y, Z;   // lattice
+  int XY;        // sites per plane
   int S;         // segments per row (X / 4)
   int R;         // rows held (TY + 4)
   int X2;        // half-row length (X / 2)
   int TY, CZ;
 };
 
+// Offsets of one thread that stay fixed while the tile walks z.
+struct LaneOff {
+  int cs;        // component stride of a half slot (R * X/2 floats)
+  int own;       // this segment's two sites in its half row (r * X/2 + 2q)
+  int ov0, ov1;  // the x neighbour outside the segment, PAR 0 (x = 4q-1) / 1 (x = 4q+4)
+  int rowbase;   // site index of the segment start within a plane (X*y + 4q)
+};
+
 // Plane index z in [-2, Z+2) wrapped onto the periodic lattice.
 __device__ __forceinline__ int wrap_z(const SlabGeom &g, int z) { return z < 0 ? z + g.Z : (z >= g.Z ? z - g.Z : z); }
 
-__device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], const SlabGeom &g, int z, int y, int q,
-                                         float4 v[3]) {
-  const int zz = wrap_z(g, z);
-  const long long off = (long long)g.X * (y + (long long)g.Y * zz) + 4 * q;
+__device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], const SlabGeom &g, const LaneOff &L,
+                                         int z, float4 v[3]) {
+  const int off = g.XY * wrap_z(g, z) + L.rowbase;  // < Nt < 2^31
 #pragma unroll
   for (int d = 0; d < 3; ++d) v[d] = *reinterpret_cast<const float4 *>(in[d] + off);
-}
-
-// Half-row address: component d, local row r, half index m.
-__device__ __forceinline__ float *half_at(float *slot, const SlabGeom &g, int d, int r, int m) {
-  return slot + ((size_t)d * g.R + r) * g.X2 + m;
 }
 
 // Stores the two sites of parity par (elements par, par+2) of the segment
@@ -78,12 +81,12 @@ __device__ __forceinline__ float pick(float a, float b, uint32_t mask) {
   return __uint_as_float((__float_as_uint(a) & ~mask) | (__float_as_uint(b) & mask));
 }
 
-__device__ __forceinline__ void put_half(float *slot, const SlabGeom &g, int r, int q, int par, const float4 v[3]) {
+__device__ __forceinline__ void put_half(float *slot, const LaneOff &L, int par, const float4 v[3]) {
   const uint32_t mask = 0u - (uint32_t)(par & 1);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float2 w = make_float2(pick(v[d].x, v[d].y, mask), pick(v[d].z, v[d].w, mask));
-    *reinterpret_cast<float2 *>(half_at(slot, g, d, r, 2 * q)) = w;
+    *reinterpret_cast<float2 *>(slot + d * L.cs + L.own) = w;
   }
 }
 
@@ -93,20 +96,21 @@ __device__ __forceinline__ void put_half(float *slot, const SlabGeom &g, int r, 
 // plane z.  Returns the accepted count; records when asked.
 template <int POT, bool RECORD, int PAR>
 __device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
-                                                float4 own[3], const float4 zm[3], const float4 zp[3], float *oth,
-                                                int r, int q, int y, int z, bool counted, const MCRecord &rec) {
+                                                const LaneOff &L, float4 own[3], const float4 zm[3],
+                                                const float4 zp[3], const float *oth, int z, bool counted,
+                                                const MCRecord &rec) {
   mcsu::PairNbrs nb;
-  const int mo = PAR == 0 ? (q == 0 ? g.X2 - 1 : 2 * q - 1) : (q == g.S - 1 ? 0 : 2 * q + 2);
+  const int mo = PAR == 0 ? L.ov0 : L.ov1;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    nb.yp[d] = *reinterpret_cast<const float2 *>(half_at(oth, g, d, r + 1, 2 * q));
-    nb.ym[d] = *reinterpret_cast<const float2 *>(half_at(oth, g, d, r - 1, 2 * q));
+    const float *sl = oth + d * L.cs;
+    nb.yp[d] = *reinterpret_cast<const float2 *>(sl + L.own + g.X2);
+    nb.ym[d] = *reinterpret_cast<const float2 *>(sl + L.own - g.X2);
     nb.zp[d] = make_float2(sel(zp[d], PAR), sel(zp[d], PAR + 2));
     nb.zm[d] = make_float2(sel(zm[d], PAR), sel(zm[d], PAR + 2));
-    nb.ov[d] = *half_at(oth, g, d, r, mo);
+    nb.ov[d] = sl[mo];
   }
-  const int zz = wrap_z(g, z);
-  const int sA = g.X * (y + g.Y * zz) + 4 * q + PAR;
+  const int sA = g.XY * wrap_z(g, z) + L.rowbase + PAR;
   mcsu::Site S[2];
   bool a[2];
   MCU4 rr;
@@ -118,12 +122,12 @@ __device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu
 
 template <int POT, bool RECORD>
 __device__ __forceinline__ uint32_t update(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
-                                           int colour, float4 own[3], const float4 zm[3], const float4 zp[3],
-                                           float *oth, int r, int q, int y, int z, bool counted,
+                                           const LaneOff &L, int colour, float4 own[3], const float4 zm[3],
+                                           const float4 zp[3], const float *oth, int y, int z, bool counted,
                                            const MCRecord &rec) {
   if (((y + z + colour) & 1) == 0)
-    return update_pair<POT, RECORD, 0>(P, kc, g, own, zm, zp, oth, r, q, y, z, counted, rec);
-  return update_pair<POT, RECORD, 1>(P, kc, g, own, zm, zp, oth, r, q, y, z, counted, rec);
+    return update_pair<POT, RECORD, 0>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
+  return update_pair<POT, RECORD, 1>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
 }
 
 __device__ __forceinline__ void copy3(float4 dst[3], const float4 src[3]) {
@@ -161,6 +165,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
   g.CZ = CZ;
   g.R = TY + 4;
   g.X2 = P.nx >> 1;
+  g.XY = P.nx * P.ny;
   const float *in[3] = {inx, iny, inz};
   float *out[3] = {outx, outy, outz};
   const mcsu::PassConst kc = mcsu::pass_const(P, st);
@@ -175,6 +180,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
   const int y = ((y0 - 2 + r) % g.Y + g.Y) % g.Y;
   const bool red_row = r >= 1 && r <= g.R - 2;
   const bool owned_row = r >= 2 && r <= g.R - 3;
+  LaneOff L;
+  L.cs = g.R * g.X2;
+  L.own = r * g.X2 + 2 * q;
+  L.ov0 = r * g.X2 + (q == 0 ? g.X2 - 1 : 2 * q - 1);
+  L.ov1 = r * g.X2 + (q == g.S - 1 ? 0 : 2 * q + 2);
+  L.rowbase = g.X * y + 4 * q;
 
   const size_t H = (size_t)3 * g.R * g.X2;  // floats per half slot
   // Bbuf(par) = lds + par*H, Rbuf(par) = lds + (2+par)*H (computed, not
@@ -188,37 +199,37 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
 
   // Prologue: planes z0-2 .. z0+2; red updates of z0-1 and z0.
   if (active) {
-    load_seg(in, g, z0 - 2, y, q, Q);
-    load_seg(in, g, z0 - 1, y, q, P0);
-    load_seg(in, g, z0, y, q, P1);
-    load_seg(in, g, z0 + 1, y, q, P2);
-    load_seg(in, g, z0 + 2, y, q, P3);
-    put_half(Bbuf((z0 - 1) & 1), g, r, q, ((y + z0 - 1 + 1) & 1), P0);
-    put_half(Bbuf(z0 & 1), g, r, q, ((y + z0 + 1) & 1), P1);
+    load_seg(in, g, L, z0 - 2, Q);
+    load_seg(in, g, L, z0 - 1, P0);
+    load_seg(in, g, L, z0, P1);
+    load_seg(in, g, L, z0 + 1, P2);
+    load_seg(in, g, L, z0 + 2, P3);
+    put_half(Bbuf((z0 - 1) & 1), L, ((y + z0 - 1 + 1) & 1), P0);
+    put_half(Bbuf(z0 & 1), L, ((y + z0 + 1) & 1), P1);
   }
   __syncthreads();
   if (active && red_row) {
-    update<POT, false>(P, kc, g, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), r, q, y, z0 - 1, false, rec);
-    acc += update<POT, RECORD>(P, kc, g, 0, P1, P0, P2, Bbuf(z0 & 1), r, q, y, z0, owned_row, rec);
-    put_half(Rbuf(z0 & 1), g, r, q, ((y + z0) & 1), P1);
+    update<POT, false>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
+    acc += update<POT, RECORD>(P, kc, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
+    put_half(Rbuf(z0 & 1), L, ((y + z0) & 1), P1);
   }
   __syncthreads();
-  if (active) put_half(Bbuf((z0 + 1) & 1), g, r, q, ((y + z0 + 1 + 1) & 1), P2);
+  if (active) put_half(Bbuf((z0 + 1) & 1), L, ((y + z0 + 1 + 1) & 1), P2);
   __syncthreads();
 
   for (int it = 0; it < CZ; ++it) {
     const int p = z0 + it;
     if (active) {
-      if (it <= CZ - 2) load_seg(in, g, p + 3, y, q, P4);
-      if (it <= CZ - 2) put_half(Bbuf(p & 1), g, r, q, ((y + p + 2 + 1) & 1), P3);
+      if (it <= CZ - 2) load_seg(in, g, L, p + 3, P4);
+      if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
       if (red_row) {
-        acc += update<POT, RECORD>(P, kc, g, 0, P2, P1, P3, Bbuf((p + 1) & 1), r, q, y, p + 1,
+        acc += update<POT, RECORD>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
                                    owned_row && it <= CZ - 2, rec);
-        if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), g, r, q, ((y + p + 1) & 1), P2);
+        if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);
       }
       if (owned_row) {
-        acc += update<POT, RECORD>(P, kc, g, 1, P1, P0, P2, Rbuf(p & 1), r, q, y, p, true, rec);
-        const long long off = (long long)g.X * (y + (long long)g.Y * p) + 4 * q;
+        acc += update<POT, RECORD>(P, kc, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
+        const int off = g.XY * p + L.rowbase;
 #pragma unroll
         for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(out[d] + off) = P1[d];
       }
