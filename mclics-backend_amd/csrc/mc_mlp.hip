@@ -66,16 +66,24 @@ struct MLPDev {
 };
 
 // Forward of one 64-site block: Xs (64x32 bf16) -> Os (64x16 FP32, columns
-// 0..3 meaningful).  H1s/H2s are 64x64 bf16 LDS scratch.  One wave.
-__device__ void forward64(const MLPDev &N, const bf16x8 w1[4], const bf16x8 w2[4][2], const bf16x8 w3[2],
-                          const __bf16 *Xs, __bf16 *H1s, __bf16 *H2s, float *Os) {
+// 0..3 meaningful).  One wave.  Hs (64x64 bf16 LDS) holds h1 and then h2:
+// each 16-row block of h2 is written over the h1 rows it was computed from,
+// after those rows were read (one wave: LDS operations complete in order).
+// Os may share its 4 KiB with Xs, which is dead after the first layer.
+// The weight operands are read from global memory tile by tile (14 KiB in
+// all, resident in L1/L2) rather than pinned in registers, which keeps the
+// kernels at 4+ waves per SIMD.
+__device__ void forward64(const MLPDev &N, const __bf16 *Xs, __bf16 *Hs, float *Os) {
+  __bf16 *const H1s = Hs;
+  __bf16 *const H2s = Hs;
   const int l = threadIdx.x & 63;
   const int r = l & 15, g = l >> 4;
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const bf16x8 a = load8(Xs + (16 * mt + r) * 32 + 8 * g);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) store_tile(H1s, 64, 16 * mt, 16 * nt, mfma(a, w1[nt], f4{0, 0, 0, 0}), N.b1, true);
+    for (int nt = 0; nt < 4; ++nt)
+      store_tile(H1s, 64, 16 * mt, 16 * nt, mfma(a, bop(N.W1b, 32, 16 * nt, 0), f4{0, 0, 0, 0}), N.b1, true);
   }
   __syncthreads();
 #pragma unroll
@@ -84,8 +92,8 @@ __device__ void forward64(const MLPDev &N, const bf16x8 w1[4], const bf16x8 w2[4
     const bf16x8 a1 = load8(H1s + (16 * mt + r) * 64 + 32 + 8 * g);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      f4 c = mfma(a0, w2[nt][0], f4{0, 0, 0, 0});
-      c = mfma(a1, w2[nt][1], c);
+      f4 c = mfma(a0, bop(N.W2b, 64, 16 * nt, 0), f4{0, 0, 0, 0});
+      c = mfma(a1, bop(N.W2b, 64, 16 * nt, 32), c);
       store_tile(H2s, 64, 16 * mt, 16 * nt, c, N.b2, true);
     }
   }
@@ -94,8 +102,8 @@ __device__ void forward64(const MLPDev &N, const bf16x8 w1[4], const bf16x8 w2[4
   for (int mt = 0; mt < 4; ++mt) {
     const bf16x8 a0 = load8(H2s + (16 * mt + r) * 64 + 8 * g);
     const bf16x8 a1 = load8(H2s + (16 * mt + r) * 64 + 32 + 8 * g);
-    f4 c = mfma(a0, w3[0], f4{0, 0, 0, 0});
-    c = mfma(a1, w3[1], c);
+    f4 c = mfma(a0, bop(N.W3b, 64, 0, 0), f4{0, 0, 0, 0});
+    c = mfma(a1, bop(N.W3b, 64, 0, 32), c);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * mt + 4 * g + i;
@@ -152,7 +160,15 @@ __device__ __forceinline__ void features(const float m[6][3], float f0, float f1
 
 __device__ __forceinline__ void write_xrow(__bf16 *Xs, int row, const float x[22], bool valid) {
 #pragma unroll
-  for (int k = 0; k < 32; ++k) Xs[row * 32 + k] = (__bf16)((valid && k < 22) ? x[k] : 0.0f);
+  for (int c = 0; c < 4; ++c) {  // four 16-byte stores
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * c + e;
+      v[e] = (__bf16)((valid && k < 22) ? x[k] : 0.0f);
+    }
+    *reinterpret_cast<bf16x8 *>(Xs + row * 32 + 8 * c) = v;
+  }
 }
 
 // log cosh(z) + log 2, FP32.
@@ -194,12 +210,12 @@ __global__ void k_mlp_pack(const float *__restrict__ P, __bf16 *W1b, __bf16 *W2b
 __global__ __launch_bounds__(64) void k_mlp_infer(MLPDev N, const float *__restrict__ X,
                                                   const float *__restrict__ base, int n, float *__restrict__ mu,
                                                   float *__restrict__ logk) {
-  __shared__ __align__(16) __bf16 Xs[64 * 32];
-  __shared__ __align__(16) __bf16 H1s[64 * 64];
-  __shared__ __align__(16) __bf16 H2s[64 * 64];
-  __shared__ float Os[64 * 16];
-  bf16x8 w1[4], w2[4][2], w3[2];
-  load_weights(N, w1, w2, w3);
+  // Xs (64x32 bf16) and Os (64x16 FP32) share 4 KiB: row l of either is
+  // lane l's own 64 bytes, and Xs is dead once the first layer has read it.
+  __shared__ __align__(16) float XOs[64 * 16];
+  __shared__ __align__(16) __bf16 Hs[64 * 64];
+  __bf16 *const Xs = reinterpret_cast<__bf16 *>(XOs);
+  float *const Os = XOs;
   const int l = threadIdx.x;
   for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
     const int s = blk * 64 + l;
@@ -209,7 +225,7 @@ __global__ __launch_bounds__(64) void k_mlp_infer(MLPDev N, const float *__restr
     for (int k = 0; k < 22; ++k) x[k] = valid ? X[(size_t)s * 22 + k] : 0.0f;
     write_xrow(Xs, l, x, valid);
     __syncthreads();
-    forward64(N, w1, w2, w3, Xs, H1s, H2s, Os);
+    forward64(N, Xs, Hs, Os);
     if (valid) {
       const float b[3] = {base[3 * (size_t)s], base[3 * (size_t)s + 1], base[3 * (size_t)s + 2]};
       float m[3], lk;
@@ -561,16 +577,16 @@ template <int POT>
 __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, float *__restrict__ nx,
                                                      float *__restrict__ ny, float *__restrict__ nz, MCDevState *st,
                                                      int colour, int finalize, MCLearnRecord rec) {
-  __shared__ __align__(16) __bf16 Xs[64 * 32];
-  __shared__ __align__(16) __bf16 H1s[64 * 64];
-  __shared__ __align__(16) __bf16 H2s[64 * 64];
-  __shared__ float Os[64 * 16];
+  // Xs (64x32 bf16) and Os (64x16 FP32) share 4 KiB: row l of either is
+  // lane l's own 64 bytes, and Xs is dead once the first layer has read it.
+  __shared__ __align__(16) float XOs[64 * 16];
+  __shared__ __align__(16) __bf16 Hs[64 * 64];
+  __bf16 *const Xs = reinterpret_cast<__bf16 *>(XOs);
+  float *const Os = XOs;
   const float T = st->T, E = st->elecE;
   const uint32_t sweep = st->sweep;
   const double cA = (double)(-P.A) * 1.5;
   const float cAf = (float)cA;
-  bf16x8 w1[4], w2[4][2], w3[2];
-  load_weights(N, w1, w2, w3);
   const int l = threadIdx.x;
   const int total = mcd_set_slots(P, 8 + colour);
   uint32_t acc = 0;
@@ -581,14 +597,20 @@ __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, fl
     const int s = valid ? i + P.nx * (j + P.ny * k) : 0;
     float m[6][3];
     bool on[6];
-    gather6(P, nx, ny, nz, i, j, k, m, on);
-    float x[22], b[3];
-    features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
-    write_xrow(Xs, l, x, valid);
+    float b[3];
+    {
+      gather6(P, nx, ny, nz, i, j, k, m, on);
+      float x[22];
+      features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
+      write_xrow(Xs, l, x, valid);
+    }
     __syncthreads();
-    forward64(N, w1, w2, w3, Xs, H1s, H2s, Os);
+    forward64(N, Xs, Hs, Os);
     float mu[3], lk;
     head(Os + l * 16, b, mu, lk);
+    // The neighbours again (cache hits) rather than 18 registers held live
+    // across the network.
+    gather6(P, nx, ny, nz, i, j, k, m, on);
     const float kappa = __expf(lk);
     const MCU4 r = mcd_philox(MCU4{(uint32_t)s, sweep, P.stream, 0x4D4Cu}, P.seed_lo, P.seed_hi);
     float q[3];
