@@ -154,6 +154,12 @@ int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series);
 int mc_site_energies(mc_handle *h, float *out);
 
 /* Lattice state transfer, AoS float[3*Nt] (src/evolve_thermal.cu:90,98). */
+/* Local order parameter of every site for the director snapshot: S of the Q
+ * tensor of the site and its active von Neumann neighbours, FP64 on the
+ * device, rounded to float; NaN where pt == 0.  Replaces the host loop of
+ * lattice_order_parameter over all sites (src/parameter_order.cpp:62-113,
+ * called from print_n, src/io.cpp:50-80).  S_out: float[Nt]. */
+int mc_local_order(mc_handle *h, float *S_out);
 int mc_get_directors_aos(mc_handle *h, float *ni_aos);
 int mc_set_directors_aos(mc_handle *h, const float *ni_aos);
 

@@ -891,6 +891,19 @@ int mc_site_energies(mc_handle *h, float *out) {
   return MC_OK;
 }
 
+int mc_local_order(mc_handle *h, float *S_out) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (!S_out) return fail(MC_EINVAL, "mc_local_order: null output");
+  rc = ensure_parity_buffers(h);
+  if (rc) return rc;
+  mc_launch_local_S(h->launch(), h->grid_for(h->P.nt), h->pbuf);
+  MC_LAUNCHED("local order parameter");
+  MC_HIP(hipMemcpyAsync(S_out, h->pbuf, (size_t)h->P.nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  MC_HIP(hipStreamSynchronize(h->stream));
+  return MC_OK;
+}
+
 int mc_get_directors_aos(mc_handle *h, float *ni_aos) {
   int rc = check_handle(h);
   if (rc) return rc;

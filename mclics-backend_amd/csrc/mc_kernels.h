@@ -74,6 +74,8 @@ void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finali
 // reduction in tile order: out[r*8..] = {E_mean, S, q6}.
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
+// Local order parameter per site (NaN where pt == 0).
+void mc_launch_local_S(const MCLaunch &L, int grid, float *out);
 // out = {E, S, q6, n_active}; series (may be null) receives {E, S}.
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series);
 void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, float *z, int n, int grid);

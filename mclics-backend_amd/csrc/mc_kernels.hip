@@ -274,6 +274,58 @@ __device__ double mcd_sym3_max(const double q[6]) {
   return fmax(l0, fmax(l1, l2));
 }
 
+// Local order parameter of every site, src/parameter_order.cpp:62-113: the
+// Q tensor of the site and its active von Neumann neighbours (wrapped on a
+// periodic axis, dropped beyond a free face), in FP64, normalised by
+// 2 * points, and S = its largest eigenvalue; NaN where pt == 0.  Neighbour
+// order as the reference's di/dj/dk loops.
+__global__ __launch_bounds__(kBlock) void k_local_S(MCDevParams P, const float *__restrict__ nx,
+                                                    const float *__restrict__ ny, const float *__restrict__ nz,
+                                                    const uint8_t *__restrict__ pt, float *__restrict__ out) {
+  const int off[7][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+  const int dims[3] = {P.nx, P.ny, P.nz};
+  for (int s = blockIdx.x * kBlock + threadIdx.x; s < P.nt; s += gridDim.x * kBlock) {
+    if (pt && pt[s] == 0) {
+      out[s] = __int_as_float(0x7fc00000);  // NaN
+      continue;
+    }
+    const int c0[3] = {s % P.nx, (s / P.nx) % P.ny, s / (P.nx * P.ny)};
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    int points = 0;
+#pragma unroll
+    for (int o = 0; o < 7; ++o) {
+      int c[3];
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        c[a] = c0[a] + off[o][a];
+        if (c[a] < 0 || c[a] >= dims[a]) {
+          if (P.per[a]) c[a] = c[a] < 0 ? c[a] + dims[a] : c[a] - dims[a];
+          else ok = false;
+        }
+      }
+      if (!ok) continue;
+      const int t = c[0] + P.nx * (c[1] + P.ny * c[2]);
+      if (pt && pt[t] == 0) continue;
+      const double x = nx[t], y = ny[t], z = nz[t];
+      q[0] += 3.0 * (x * x);
+      q[1] += 3.0 * (x * y);
+      q[2] += 3.0 * (x * z);
+      q[3] += 3.0 * (y * y);
+      q[4] += 3.0 * (y * z);
+      q[5] += 3.0 * (z * z);
+      q[0] -= 1.0;
+      q[3] -= 1.0;
+      q[5] -= 1.0;
+      ++points;
+    }
+    const double norm = 2.0 * points;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) q[c] /= norm;
+    out[s] = (float)mcd_sym3_max(q);
+  }
+}
+
 // Second stage, one workgroup, fixed summation order (deterministic):
 // out = {E_mean, S, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n_active}.
 __global__ __launch_bounds__(kBlock) void k_measure_final(const double *__restrict__ partials, int nparts,
@@ -484,6 +536,10 @@ void mc_launch_site_energy(const MCLaunch &L, int grid, float *out) {
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series) {
   MC_DISPATCH(k_measure, grid, L.P, L.nx, L.ny, L.nz, L.pt, L.sx, L.sy, L.sz, L.st, partials);
   k_measure_final<<<1, kBlock, 0, L.stream>>>(partials, grid, out, series);
+}
+
+void mc_launch_local_S(const MCLaunch &L, int grid, float *out) {
+  k_local_S<<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.pt, out);
 }
 
 void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, float *z, int n, int grid) {

@@ -65,50 +65,64 @@ struct MLPDev {
   const float *b1, *b2, *b3;
 };
 
-// Forward of one 64-site block: Xs (64x32 bf16) -> Os (64x16 FP32, columns
-// 0..3 meaningful).  One wave.  Hs (64x64 bf16 LDS) holds h1 and then h2:
-// each 16-row block of h2 is written over the h1 rows it was computed from,
-// after those rows were read (one wave: LDS operations complete in order).
-// Os may share its 4 KiB with Xs, which is dead after the first layer.
-// The weight operands are read from global memory tile by tile (14 KiB in
-// all, resident in L1/L2) rather than pinned in registers, which keeps the
-// kernels at 4+ waves per SIMD.
-__device__ void forward64(const MLPDev &N, const __bf16 *Xs, __bf16 *Hs, float *Os) {
-  __bf16 *const H1s = Hs;
-  __bf16 *const H2s = Hs;
+// Forward of one 64-site block in transposed form: every layer computes
+// D = W (units x K) * H^T (K x sites), weights as the A operand, sites as B.
+// Lane (r, g) = (l % 16, l / 16) then holds units 16 mt + 4 g + i (i = 0..3)
+// of site r of each 16-site tile, and the eight values it holds for tiles
+// 2c and 2c+1 (units 32c + 4g + 0..3 and 32c + 16 + 4g + 0..3), after bias,
+// ReLU and rounding to bf16, are exactly its B operand for the next layer's
+// K chunk c, provided the weight operand uses the same K order (aop_hidden).
+// So the hidden activations never leave registers; only the input rows
+// (Xs, 64 x 32 bf16, row = site) and the four outputs per site (Os, 64 x 4
+// FP32) go through LDS.  One wave.
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 aop_hidden(const __bf16 *W, int ld, int row, int c, int g) {
+  const __bf16 *p = W + (size_t)row * ld + 32 * c + 4 * g;
+  const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(p);
+  const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(p + 16);
+  return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// ReLU, bf16 rounding and the D-tile to B-operand packing of K chunk c.
+__device__ __forceinline__ bf16x8 pack_hidden(const f4 &a, const f4 &b) {
+  bf16x8 v;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = (__bf16)fmaxf(a[e], 0.0f);
+    v[4 + e] = (__bf16)fmaxf(b[e], 0.0f);
+  }
+  return v;
+}
+
+// The weight operands (rows 16 mt + l%16 of W1/W2, rows l%16 of W3; 14 KiB in
+// all, L1-resident) are read where they are used rather than pinned in 56
+// registers for the kernel's lifetime.
+__device__ void forward64t(const MLPDev &N, const __bf16 *Xs, float *Os) {
+  // Compiler-only barrier: keeps the weight loads here instead of hoisted
+  // out of the caller's site loop into 56 long-lived registers.
+  asm volatile("" ::: "memory");
   const int l = threadIdx.x & 63;
   const int r = l & 15, g = l >> 4;
+#pragma unroll 1
+  for (int t = 0; t < 4; ++t) {
+    const bf16x8 x = load8(Xs + (16 * t + r) * 32 + 8 * g);
+    f4 h1[4];
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8 a = load8(Xs + (16 * mt + r) * 32 + 8 * g);
+    for (int mt = 0; mt < 4; ++mt)
+      h1[mt] = mfma(load8(N.W1b + (16 * mt + r) * 32 + 8 * g), x, *reinterpret_cast<const f4 *>(N.b1 + 16 * mt + 4 * g));
+    const bf16x8 a0 = pack_hidden(h1[0], h1[1]), a1 = pack_hidden(h1[2], h1[3]);
+    f4 h2[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-      store_tile(H1s, 64, 16 * mt, 16 * nt, mfma(a, bop(N.W1b, 32, 16 * nt, 0), f4{0, 0, 0, 0}), N.b1, true);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8 a0 = load8(H1s + (16 * mt + r) * 64 + 8 * g);
-    const bf16x8 a1 = load8(H1s + (16 * mt + r) * 64 + 32 + 8 * g);
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      f4 c = mfma(a0, bop(N.W2b, 64, 16 * nt, 0), f4{0, 0, 0, 0});
-      c = mfma(a1, bop(N.W2b, 64, 16 * nt, 32), c);
-      store_tile(H2s, 64, 16 * mt, 16 * nt, c, N.b2, true);
+    for (int mt = 0; mt < 4; ++mt) {
+      f4 c = mfma(aop_hidden(N.W2b, 64, 16 * mt + r, 0, g), a0, *reinterpret_cast<const f4 *>(N.b2 + 16 * mt + 4 * g));
+      h2[mt] = mfma(aop_hidden(N.W2b, 64, 16 * mt + r, 1, g), a1, c);
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8 a0 = load8(H2s + (16 * mt + r) * 64 + 8 * g);
-    const bf16x8 a1 = load8(H2s + (16 * mt + r) * 64 + 32 + 8 * g);
-    f4 c = mfma(a0, bop(N.W3b, 64, 0, 0), f4{0, 0, 0, 0});
-    c = mfma(a1, bop(N.W3b, 64, 0, 32), c);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = 16 * mt + 4 * g + i;
-      Os[row * 16 + r] = c[i] + (r < 4 ? N.b3[r] : 0.0f);
-    }
+    const bf16x8 b0 = pack_hidden(h2[0], h2[1]), b1 = pack_hidden(h2[2], h2[3]);
+    f4 o = g == 0 ? *reinterpret_cast<const f4 *>(N.b3) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+    o = mfma(aop_hidden(N.W3b, 64, r, 0, g), b0, o);
+    o = mfma(aop_hidden(N.W3b, 64, r, 1, g), b1, o);
+    if (g == 0) *reinterpret_cast<f4 *>(Os + (16 * t + r) * 4) = o;  // outputs 0..3 of site 16t + r
   }
   __syncthreads();
 }
@@ -210,12 +224,9 @@ __global__ void k_mlp_pack(const float *__restrict__ P, __bf16 *W1b, __bf16 *W2b
 __global__ __launch_bounds__(64) void k_mlp_infer(MLPDev N, const float *__restrict__ X,
                                                   const float *__restrict__ base, int n, float *__restrict__ mu,
                                                   float *__restrict__ logk) {
-  // Xs (64x32 bf16) and Os (64x16 FP32) share 4 KiB: row l of either is
-  // lane l's own 64 bytes, and Xs is dead once the first layer has read it.
-  __shared__ __align__(16) float XOs[64 * 16];
-  __shared__ __align__(16) __bf16 Hs[64 * 64];
-  __bf16 *const Xs = reinterpret_cast<__bf16 *>(XOs);
-  float *const Os = XOs;
+  __shared__ __align__(16) __bf16 Xs[64 * 32];
+  __shared__ __align__(16) float Os[64 * 4];
+
   const int l = threadIdx.x;
   for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
     const int s = blk * 64 + l;
@@ -225,11 +236,11 @@ __global__ __launch_bounds__(64) void k_mlp_infer(MLPDev N, const float *__restr
     for (int k = 0; k < 22; ++k) x[k] = valid ? X[(size_t)s * 22 + k] : 0.0f;
     write_xrow(Xs, l, x, valid);
     __syncthreads();
-    forward64(N, Xs, Hs, Os);
+    forward64t(N, Xs, Os);
     if (valid) {
       const float b[3] = {base[3 * (size_t)s], base[3 * (size_t)s + 1], base[3 * (size_t)s + 2]};
       float m[3], lk;
-      head(Os + l * 16, b, m, lk);
+      head(Os + l * 4, b, m, lk);
       mu[3 * (size_t)s] = m[0];
       mu[3 * (size_t)s + 1] = m[1];
       mu[3 * (size_t)s + 2] = m[2];
@@ -577,12 +588,9 @@ template <int POT>
 __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, float *__restrict__ nx,
                                                      float *__restrict__ ny, float *__restrict__ nz, MCDevState *st,
                                                      int colour, int finalize, MCLearnRecord rec) {
-  // Xs (64x32 bf16) and Os (64x16 FP32) share 4 KiB: row l of either is
-  // lane l's own 64 bytes, and Xs is dead once the first layer has read it.
-  __shared__ __align__(16) float XOs[64 * 16];
-  __shared__ __align__(16) __bf16 Hs[64 * 64];
-  __bf16 *const Xs = reinterpret_cast<__bf16 *>(XOs);
-  float *const Os = XOs;
+  __shared__ __align__(16) __bf16 Xs[64 * 32];
+  __shared__ __align__(16) float Os[64 * 4];
+
   const float T = st->T, E = st->elecE;
   const uint32_t sweep = st->sweep;
   const double cA = (double)(-P.A) * 1.5;
@@ -605,9 +613,9 @@ __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, fl
       write_xrow(Xs, l, x, valid);
     }
     __syncthreads();
-    forward64(N, Xs, Hs, Os);
+    forward64t(N, Xs, Os);
     float mu[3], lk;
-    head(Os + l * 16, b, mu, lk);
+    head(Os + l * 4, b, mu, lk);
     // The neighbours again (cache hits) rather than 18 registers held live
     // across the network.
     gather6(P, nx, ny, nz, i, j, k, m, on);

@@ -231,7 +231,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
         acc += update<POT, RECORD>(P, kc, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
         const int off = g.XY * p + L.rowbase;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(out[d] + off) = P1[d];
+        for (int d = 0; d < 3; ++d) {  // streaming stores: nothing reads them before the next launch
+          typedef float nf4 __attribute__((ext_vector_type(4)));
+          const nf4 v = {P1[d].x, P1[d].y, P1[d].z, P1[d].w};
+          __builtin_nontemporal_store(v, reinterpret_cast<nf4 *>(out[d] + off));
+        }
       }
     }
     __syncthreads();

@@ -157,7 +157,11 @@ void EvolveNGPU::fetch() { check(mc_get_directors_aos(h, ni), "mc_get_directors_
 
 void EvolveNGPU::snapshot(const std::string &fname) {
   fetch();
-  print_n(fname.c_str(), ni, *params, pt, geometry->bound);
+  // The local order parameter of every site comes from the device (one
+  // kernel) instead of Nt host eigen-solves.
+  std::vector<float> S((size_t)Nx * Ny * Nz);
+  check(mc_local_order(h, S.data()), "mc_local_order");
+  print_n(fname.c_str(), ni, *params, pt, geometry->bound, S.data());
 }
 
 FILE *EvolveNGPU::open_po(const char *first_col) {
@@ -306,7 +310,8 @@ double lattice_order_parameter(const float *ni, const int *pt, int i, int j, int
   return sym3_max(Q);
 }
 
-int print_n(const char *fname, const float *ni, const Parameters &p, const int *pt, const int bound[3]) {
+int print_n(const char *fname, const float *ni, const Parameters &p, const int *pt, const int bound[3],
+            const float *S_site) {
   FILE *out = fopen(fname, "w");
   if (!out) {
     perror(fname);
@@ -329,7 +334,8 @@ int print_n(const char *fname, const float *ni, const Parameters &p, const int *
       const long long a = c * chunk, e = std::min(nt, a + chunk);
       for (long long s = a; s < e; ++s) {
         const int i = (int)(s % p.Nx), j = (int)((s / p.Nx) % p.Ny), k = (int)(s / ((long long)p.Nx * p.Ny));
-        const float S = pt[s] ? (float)lattice_order_parameter(ni, pt, i, j, k, p, bound) : (float)NAN;
+        const float S = !pt[s] ? (float)NAN
+                        : S_site ? S_site[s] : (float)lattice_order_parameter(ni, pt, i, j, k, p, bound);
         const int n = snprintf(line, sizeof(line), "%d,%d,%d,%g,%g,%g,%g,%d\n", i, j, k, (double)ni[3 * s],
                                (double)ni[3 * s + 1], (double)ni[3 * s + 2], (double)S, pt[s]);
         b.append(line, n);

@@ -78,7 +78,10 @@ class electricEvolveNGPU : public EvolveNGPU {
 
 // Writes a director snapshot (src/io.cpp:205-235): "x,y,z,nx,ny,nz,S,pt"
 // with the local order parameter of each active site and NaN elsewhere.
-int print_n(const char *fname, const float *ni, const Parameters &params, const int *pt, const int bound[3]);
+// S_site, when given, holds those values (from mc_local_order); otherwise
+// they are computed here.
+int print_n(const char *fname, const float *ni, const Parameters &params, const int *pt, const int bound[3],
+            const float *S_site = nullptr);
 
 // Local order parameter of site (i,j,k) and its active von Neumann
 // neighbours (src/parameter_order.cpp:62-113), FP64.

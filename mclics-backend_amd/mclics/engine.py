@@ -52,6 +52,7 @@ EXPORTS = (
     "mc_measure",
     "mc_measure_series",
     "mc_site_energies",
+    "mc_local_order",
     "mc_get_directors_aos",
     "mc_set_directors_aos",
     "mc_get_acceptance",
@@ -162,6 +163,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_measure_series.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_site_energies.argtypes = [vp, f32p]
+    lib.mc_local_order.argtypes = [vp, f32p]
     lib.mc_get_directors_aos.argtypes = [vp, f32p]
     lib.mc_set_directors_aos.argtypes = [vp, f32p]
     lib.mc_get_acceptance.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -331,6 +333,12 @@ class Lattice:
         a, s = C.c_uint32(), C.c_uint32()
         _check(self._lib.mc_get_acceptance(self._h, C.byref(a), C.byref(s)))
         return a.value, s.value
+
+    def local_order(self) -> np.ndarray:
+        """Local S of every site (NaN where pt == 0), as in the snapshot files."""
+        out = np.empty(self.nt, np.float32)
+        _check(self._lib.mc_local_order(self._h, _f32(out)))
+        return out
 
     def directors(self) -> np.ndarray:
         out = np.empty(3 * self.nt, np.float32)
