@@ -56,7 +56,7 @@ __global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var
   st->acc_count = 0u;
   st->blocks_done = 0u;
   st->last_acc = 0u;
-  st->pad = 0u;
+  st->meas_done = 0u;
 }
 
 }  // namespace
@@ -260,6 +260,17 @@ int enqueue_sweep(mc_handle *h, const MCRecord *rec, const MCLearnRecord *lrec =
   return MC_OK;
 }
 
+// E, S and Q of the current lattice into h->result (and series when given):
+// one segment-kernel launch for a bulk periodic lattice, else the general
+// per-site kernel and its one-block finisher.
+void launch_measure(mc_handle *h, double *series) {
+  const MCLaunch L = h->launch();
+  if (mc_x4_eligible(L))  // any sweep order: the measurement reads the lattice only
+    mc_launch_measure_x4(L, h->measure_grid, h->measure_grid % 8 == 0 ? 1 : 0, h->partials, h->result, series);
+  else
+    mc_launch_measure(L, h->measure_grid, h->partials, h->result, series);
+}
+
 // Graph of n plain sweeps starting from the current plane set.  Capture
 // records the launches without running them, so the host-side plane swaps
 // and sweep count made while capturing are undone afterwards.
@@ -337,6 +348,11 @@ const char *mc_to_dev_params(const mc_params *p, MCDevParams *out) {
     P.surf_W[s] = s < p->n_surf ? p->surf_W[s] : 0.0f;
     for (int c = 0; c < 3; ++c) P.surf_dir[s][c] = s < p->n_surf ? p->surf_dir[s][c] : 0.0f;
   }
+  const uint32_t hx = (uint32_t)(p->nx + 1) / 2u, hy = (uint32_t)(p->ny + 1) / 2u;
+  P.div_hx = mcd_div_make(hx);
+  P.div_ny = mcd_div_make((uint32_t)p->ny);
+  P.div_nj = mcd_div_make(hy);
+  P.div_ninj = mcd_div_make(hx * hy);
   P.seed_lo = (uint32_t)(p->seed & 0xFFFFFFFFu);
   P.seed_hi = (uint32_t)(p->seed >> 32);
   P.stream = p->stream;
@@ -838,7 +854,7 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
 int mc_measure(mc_handle *h, double *E, double *S, double *q6) {
   int rc = check_handle(h);
   if (rc) return rc;
-  mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result, nullptr);
+  launch_measure(h, nullptr);
   MC_LAUNCHED("measure");
   double r[9];
   MC_HIP(hipMemcpyAsync(r, h->result, sizeof(r), hipMemcpyDeviceToHost, h->stream));
@@ -865,7 +881,7 @@ int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series) {
     rc = enqueue_sweep(h, nullptr);
     if (rc) return rc;
     // (re-read after the sweep: the slab kernel moves the lattice to the other plane set)
-    mc_launch_measure(h->launch(), h->measure_grid, h->partials, h->result, h->series + 2 * (size_t)w);
+    launch_measure(h, h->series + 2 * (size_t)w);
   }
   MC_LAUNCHED("measured sweep");
   std::vector<double> buf(2 * (size_t)n);

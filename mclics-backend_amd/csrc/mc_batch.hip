@@ -40,7 +40,7 @@ __global__ void k_batch_init(MCDevState *st, int nrep, const float *T, const flo
   st[r].acc_count = 0u;
   st[r].blocks_done = 0u;
   st[r].last_acc = 0u;
-  st[r].pad = 0u;
+  st[r].meas_done = 0u;
 }
 
 __global__ void k_batch_set_T(MCDevState *st, int r, float T) { st[r].T = T; }

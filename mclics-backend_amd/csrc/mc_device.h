@@ -9,6 +9,7 @@
 // unpacked FP32, so keeping the reference's mixed precision costs conversion
 // instructions only.
 #pragma once
+#include <stddef.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,6 +29,30 @@ enum {
 };
 
 // Immutable per-handle parameters, passed by value as a kernel argument.
+// Exact unsigned division by a divisor fixed for a launch, with one
+// multiply-high (Granlund & Montgomery, PLDI 1994, round-up method): for
+// 32-bit n, q = (t + ((n - t) >> 1)) >> (l - 1) with t = mulhi(m, n),
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1; d = 1 gives q = n.
+struct MCDiv {
+  uint32_t m, sh, d;
+};
+
+__host__ __device__ inline MCDiv mcd_div_make(uint32_t d) {
+  MCDiv r{0u, 0u, d};
+  if (d <= 1u) return r;
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  r.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1ull);
+  r.sh = l - 1;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t mcd_div(uint32_t n, const MCDiv &D) {
+  if (D.d <= 1u) return n;
+  const uint32_t t = __umulhi(D.m, n);
+  return (t + ((n - t) >> 1)) >> D.sh;
+}
+
 struct MCDevParams {
   int nx, ny, nz;
   int per[3];  // 1 periodic, 0 free
@@ -42,6 +67,8 @@ struct MCDevParams {
   uint32_t seed_lo, seed_hi, stream;
   int nt;
   int valid;
+  // dividers of the site-set maps: (nx+1)/2, ny, Ni = (nx+1)/2, Nj = (ny+1)/2, Ni*Nj
+  MCDiv div_hx, div_ny, div_nj, div_ninj;
 };
 
 // Mutable sweep state in device memory, so captured graphs replay with the
@@ -51,11 +78,13 @@ struct MCDevState {
   float elecE;
   float ang_var;
   uint32_t sweep;        // index of the next sweep (RNG counter word 1)
-  uint32_t acc_count;    // accepted moves of the sweep in progress
-  uint32_t blocks_done;  // last-block detection of the final pass
+  uint32_t acc_count;    // accepted moves of the sweep in progress (8-byte aligned: low word of
+  uint32_t blocks_done;  //   one 64-bit counter with blocks_done, last-block detection of the final pass)
   uint32_t last_acc;     // accepted moves of the last completed sweep
-  uint32_t pad;
+  uint32_t meas_done;    // spare (zero)
 };
+static_assert(offsetof(MCDevState, acc_count) % 8 == 0 && offsetof(MCDevState, blocks_done) == offsetof(MCDevState, acc_count) + 4,
+              "acc_count/blocks_done form one 64-bit counter");
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11): two 32x32->64 products per round,
@@ -95,19 +124,19 @@ __device__ __forceinline__ float mcd_u01(uint32_t x) { return (float)(x >> 8) * 
 __device__ __forceinline__ bool mcd_set_site(const MCDevParams &P, int set, int t, int &i, int &j, int &k) {
   if (set >= 8) {
     const int c = set - 8;
-    const int hx = (P.nx + 1) >> 1;
-    const int h = t % hx;
-    const int row = t / hx;
-    j = row % P.ny;
-    k = row / P.ny;
+    const int row = (int)mcd_div((uint32_t)t, P.div_hx);
+    const int h = t - row * (int)P.div_hx.d;
+    k = (int)mcd_div((uint32_t)row, P.div_ny);
+    j = row - k * P.ny;
     i = 2 * h + ((j + k + c) & 1);
     return i < P.nx && k < P.nz;
   }
   const int di = set & 1, dj = (set >> 1) & 1, dk = set >> 2;
-  const int Ni = (P.nx + 1) >> 1, Nj = (P.ny + 1) >> 1;
-  i = di + 2 * (t % Ni);
-  j = dj + 2 * ((t / Ni) % Nj);
-  k = dk + 2 * (t / (Ni * Nj));
+  const int r1 = (int)mcd_div((uint32_t)t, P.div_hx);  // t / Ni (Ni = hx)
+  const int kk = (int)mcd_div((uint32_t)t, P.div_ninj);
+  i = di + 2 * (t - r1 * (int)P.div_hx.d);
+  j = dj + 2 * (r1 - (int)mcd_div((uint32_t)r1, P.div_nj) * (int)P.div_nj.d);
+  k = dk + 2 * kk;
   return i < P.nx && j < P.ny && k < P.nz;
 }
 
@@ -300,15 +329,24 @@ __device__ __forceinline__ uint32_t mcd_block_sum(uint32_t v) {
 
 // Device-side ang_var adaptation by the last workgroup of a sweep's final
 // pass, src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
+//
+// acc_count (low word) and blocks_done (high word) are adjacent in
+// MCDevState, so a final-pass workgroup adds (1 << 32) | count in ONE 64-bit
+// atomic: the value it gets back tells it whether it is the last one and
+// already holds every other workgroup's count, so no memory fence is needed
+// (an agent-scope fence in every workgroup costs an L2 write-back each).
+// Earlier passes of the sweep add their counts to the low word; they ended
+// with their kernel, so the final pass's atomics see them.
 __device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, uint32_t block_acc, bool finalize) {
   if (threadIdx.x != 0) return;
-  atomicAdd(&st->acc_count, block_acc);
-  if (!finalize) return;
-  __threadfence();
-  const uint32_t done = atomicAdd(&st->blocks_done, 1u);
-  if (done != gridDim.x - 1) return;
-  __threadfence();
-  const uint32_t acc = atomicAdd(&st->acc_count, 0u);
+  if (!finalize) {
+    atomicAdd(&st->acc_count, block_acc);
+    return;
+  }
+  unsigned long long *word = reinterpret_cast<unsigned long long *>(&st->acc_count);
+  const unsigned long long old = atomicAdd(word, (1ull << 32) | (unsigned long long)block_acc);
+  if ((uint32_t)(old >> 32) != gridDim.x - 1) return;
+  const uint32_t acc = (uint32_t)old + block_acc;
   float a = st->ang_var;
   const double rate = 1.0 * (double)acc / (double)P.valid;
   if (rate < 0.5) {
@@ -320,11 +358,28 @@ __device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, ui
   }
   st->ang_var = a;
   st->last_acc = acc;
-  st->acc_count = 0u;
-  st->blocks_done = 0u;
+  *word = 0ull;  // acc_count = blocks_done = 0 for the next sweep
   st->sweep = st->sweep + 1u;
 }
 
+// Symmetric 3x3 eigenvalue closed form (Smith, CACM 4(4) 1961); largest.
+__device__ inline double mcd_sym3_max(const double q[6]) {
+  const double a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+  const double p1 = b * b + c * c + e * e;
+  const double tr = (a + d + f) / 3.0;
+  const double p2 = (a - tr) * (a - tr) + (d - tr) * (d - tr) + (f - tr) * (f - tr) + 2.0 * p1;
+  if (p2 <= 1e-300) return tr;
+  const double pp = sqrt(p2 / 6.0);
+  const double B0 = (a - tr) / pp, B1 = b / pp, B2 = c / pp, B3 = (d - tr) / pp, B4 = e / pp, B5 = (f - tr) / pp;
+  const double det = B0 * (B3 * B5 - B4 * B4) - B1 * (B1 * B5 - B4 * B2) + B2 * (B1 * B4 - B3 * B2);
+  double r = det / 2.0;
+  r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+  const double phi = acos(r) / 3.0;
+  const double l0 = tr + 2.0 * pp * cos(phi);
+  const double l2 = tr + 2.0 * pp * cos(phi + 2.0 * 3.14159265358979323846 / 3.0);
+  const double l1 = 3.0 * tr - l0 - l2;
+  return fmax(l0, fmax(l1, l2));
+}
 
 // Tile schedule of the persistent sweep kernels.  Tiles are the 256-item
 // units one workgroup handles per iteration.  xcd == 0: plain grid stride.

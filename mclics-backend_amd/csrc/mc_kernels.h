@@ -42,6 +42,8 @@ void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *pr
 // Fast pass for all-active periodic lattices with Nx % 4 == 0 (mc_sweep_x4.hip).
 bool mc_x4_eligible(const MCLaunch &L);
 int mc_x4_slots(const MCDevParams &P);
+// Measurement of such a lattice in one launch (mc_sweep_x4.hip); out/series as mc_launch_measure.
+void mc_launch_measure_x4(const MCLaunch &L, int grid, int xcd, double *partials, double *out, double *series);
 void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec);
 // Slab kernel (mc_sweep_slab.hip): one launch per sweep, both colours, reading
 // the lattice from L.nx.. and writing it to the o* planes.  Same eligibility
@@ -78,6 +80,8 @@ void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
 void mc_launch_local_S(const MCLaunch &L, int grid, float *out);
 // out = {E, S, q6, n_active}; series (may be null) receives {E, S}.
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series);
+// One-block reduction of nparts x 8 partials into out (and series).
+void mc_launch_measure_final(hipStream_t stream, const double *partials, int nparts, double *out, double *series);
 void mc_launch_aos_to_soa(hipStream_t s, const float *aos, float *x, float *y, float *z, int n, int grid);
 void mc_launch_soa_to_aos(hipStream_t s, const float *x, const float *y, const float *z, float *aos, int n,
                           int grid);

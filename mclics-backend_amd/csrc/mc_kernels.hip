@@ -255,24 +255,6 @@ __global__ __launch_bounds__(kBlock) void k_measure(MCDevParams P, const float *
   }
 }
 
-// Symmetric 3x3 eigenvalue closed form (Smith, CACM 4(4) 1961); largest.
-__device__ double mcd_sym3_max(const double q[6]) {
-  const double a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
-  const double p1 = b * b + c * c + e * e;
-  const double tr = (a + d + f) / 3.0;
-  const double p2 = (a - tr) * (a - tr) + (d - tr) * (d - tr) + (f - tr) * (f - tr) + 2.0 * p1;
-  if (p2 <= 1e-300) return tr;
-  const double pp = sqrt(p2 / 6.0);
-  const double B0 = (a - tr) / pp, B1 = b / pp, B2 = c / pp, B3 = (d - tr) / pp, B4 = e / pp, B5 = (f - tr) / pp;
-  const double det = B0 * (B3 * B5 - B4 * B4) - B1 * (B1 * B5 - B4 * B2) + B2 * (B1 * B4 - B3 * B2);
-  double r = det / 2.0;
-  r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
-  const double phi = acos(r) / 3.0;
-  const double l0 = tr + 2.0 * pp * cos(phi);
-  const double l2 = tr + 2.0 * pp * cos(phi + 2.0 * 3.14159265358979323846 / 3.0);
-  const double l1 = 3.0 * tr - l0 - l2;
-  return fmax(l0, fmax(l1, l2));
-}
 
 // Local order parameter of every site, src/parameter_order.cpp:62-113: the
 // Q tensor of the site and its active von Neumann neighbours (wrapped on a
@@ -531,6 +513,10 @@ void mc_launch_pass_inject(const MCLaunch &L, int set, int grid, const float *pr
 
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out) {
   MC_DISPATCH(k_site_energy, grid, L.P, L.nx, L.ny, L.nz, L.pt, L.sx, L.sy, L.sz, L.st, out);
+}
+
+void mc_launch_measure_final(hipStream_t stream, const double *partials, int nparts, double *out, double *series) {
+  k_measure_final<<<1, kBlock, 0, stream>>>(partials, nparts, out, series);
 }
 
 void mc_launch_measure(const MCLaunch &L, int grid, double *partials, double *out, double *series) {

@@ -20,6 +20,25 @@ namespace {
 constexpr int kBlock = MCD_BLOCK;
 using mcsu::sel;
 
+// Row of segment-row index rowi.  With fewer than 64 segments per row a wave
+// spans several rows, and rows of opposite (j + k) parity take the two
+// different PAR branches; so the rows with (j + k) even come first, then the
+// odd ones, and every wave stays on one parity (when (Ny Nz / 2) * Nx/4 is a
+// multiple of 64).  Ny is even (periodic axes are).  The generator is keyed
+// by site, so the order changes nothing but the speed.
+__device__ __forceinline__ void x4_row(int rowi, int nseg, int Y, int Z, int &j, int &k) {
+  if (nseg >= 64) {
+    j = rowi % Y;
+    k = rowi / Y;
+    return;
+  }
+  const int h = (Y * Z) >> 1, hy = Y >> 1;
+  const int odd = rowi >= h ? 1 : 0;
+  const int e = rowi - odd * h;
+  k = e / hy;
+  j = 2 * (e - k * hy) + ((k + odd) & 1);
+}
+
 // One thread's segment update; PAR is the x parity of the first site of the
 // pass colour in this row (sites at 4q+PAR and 4q+PAR+2).
 template <int POT, bool RECORD, int PAR>
@@ -80,7 +99,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
     if (t >= total) continue;
     const int q = t % nseg;
     const int rowi = t / nseg;
-    const int j = rowi % P.ny, k = rowi / P.ny;
+    int j, k;
+    x4_row(rowi, nseg, P.ny, P.nz, j, k);
     if (((j + k + colour) & 1) == 0)
       acc += segment<POT, RECORD, 0>(P, kc, nx, ny, nz, q, j, k, rec);
     else
@@ -88,6 +108,94 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
   }
   const uint32_t bsum = mcd_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, finalize != 0);
+}
+
+// Measurement of an all-active periodic lattice with Nx % 4 == 0: mean site
+// energy (each site's energy with the reference arithmetic, src/geometry.cpp:
+// 19-72, summed in FP64) and the FP64 Q tensor, one 16-byte segment per thread
+// over the same XCD-contiguous tile order as the sweeps, so the y/z
+// neighbour rows come from L2.  Writes FP64 partials per workgroup
+// {E, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n}; the one-block finisher of the general
+// measurement reduces them in workgroup order (deterministic).
+template <int POT>
+__global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const float *__restrict__ nx,
+                                                       const float *__restrict__ ny, const float *__restrict__ nz,
+                                                       const MCDevState *st, int xcd,
+                                                       double *__restrict__ partials) {
+  const float elecE = st->elecE;
+  const double cA = (double)(-P.A) * 1.5;
+  const int X = P.nx, Y = P.ny, Z = P.nz;
+  const int nseg = X >> 2;
+  const int total = nseg * Y * Z;
+  const float *pl[3] = {nx, ny, nz};
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const MCTileRange tr = mcd_tiles((total + kBlock - 1) / kBlock, xcd);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int t = tile * kBlock + threadIdx.x;
+    if (t >= total) continue;
+    const int q = t % nseg, rowi = t / nseg;
+    const int j = rowi % Y, k = rowi / Y;
+    const int jp = j + 1 == Y ? 0 : j + 1, jm = j == 0 ? Y - 1 : j - 1;
+    const int kp = k + 1 == Z ? 0 : k + 1, km = k == 0 ? Z - 1 : k - 1;
+    const int row = X * (j + Y * k), x0 = 4 * q;
+    const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
+    const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
+    const int xl = row + (q == 0 ? X - 1 : x0 - 1), xr = row + (q == nseg - 1 ? 0 : x0 + 4);
+    float4 c[3], yp[3], ym[3], zp[3], zm[3];
+    float ol[3], orr[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      c[d] = *reinterpret_cast<const float4 *>(pl[d] + row + x0);
+      yp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
+      ym[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
+      zp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
+      zm[d] = *reinterpret_cast<const float4 *>(pl[d] + r_km);
+      ol[d] = pl[d][xl];
+      orr[d] = pl[d][xr];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float o[3], m[6][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        o[d] = sel(c[d], e);
+        m[0][d] = e == 3 ? orr[d] : sel(c[d], e + 1);  // +x
+        m[1][d] = e == 0 ? ol[d] : sel(c[d], e - 1);   // -x
+        m[2][d] = sel(yp[d], e);
+        m[3][d] = sel(ym[d], e);
+        m[4][d] = sel(zp[d], e);
+        m[5][d] = sel(zm[d], e);
+      }
+      double so = 0.0;
+#pragma unroll
+      for (int b = 0; b < 6; ++b) so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+      float E = (float)so;
+      if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
+      const double x = o[0], y = o[1], z = o[2];
+      a[0] += (double)E;
+      a[1] += 3.0 * x * x - 1.0;
+      a[2] += 3.0 * x * y;
+      a[3] += 3.0 * x * z;
+      a[4] += 3.0 * y * y - 1.0;
+      a[5] += 3.0 * y * z;
+      a[6] += 3.0 * z * z - 1.0;
+      a[7] += 1.0;
+    }
+  }
+  __shared__ double red[8][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double v = a[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double v = 0;
+    for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
+    partials[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+  }
 }
 
 template <int POT>
@@ -106,6 +214,21 @@ bool mc_x4_eligible(const MCLaunch &L) {
 }
 
 int mc_x4_slots(const MCDevParams &P) { return (P.nx / 4) * P.ny * P.nz; }
+
+void mc_launch_measure_x4(const MCLaunch &L, int grid, int xcd, double *partials, double *out, double *series) {
+  switch (L.P.potential) {
+    case MCD_GHRL:
+      k_measure_x4<MCD_GHRL><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      break;
+    case MCD_PEAR:
+      k_measure_x4<MCD_PEAR><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      break;
+    default:
+      k_measure_x4<MCD_LL><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      break;
+  }
+  mc_launch_measure_final(L.stream, partials, grid, out, series);
+}
 
 void mc_launch_sweep_x4(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
   switch (L.P.potential) {
@@ -138,8 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
                                                      const int *__restrict__ tile_rep, const int *__restrict__ rep_tile0,
                                                      const long long *__restrict__ rep_off, int ntiles, int nrep,
                                                      float *__restrict__ nx, float *__restrict__ ny,
-                                                     float *__restrict__ nz, int colour, int finalize, int xcd,
-                                                     uint32_t *blocks_done) {
+                                                     float *__restrict__ nz, int colour, int xcd) {
   const MCTileRange tr = mcd_tiles(ntiles, xcd);
   for (int tile = tr.first; tile < tr.end; tile += tr.step) {
     const int r = tile_rep[tile];
@@ -154,7 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
     if (t < total) {
       const int q = t % nseg;
       const int rowi = t / nseg;
-      const int j = rowi % P.ny, k = rowi / P.ny;
+      int j, k;
+      x4_row(rowi, nseg, P.ny, P.nz, j, k);
       if (((j + k + colour) & 1) == 0)
         acc = segment<POT, false, 0>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
       else
@@ -164,18 +287,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
     if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
     __syncthreads();  // wsum of mcd_block_sum is reused by the next tile
   }
-  if (!finalize) return;
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(blocks_done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// End of a batched sweep: per-replica ang_var adaptation from the counts the
+// passes accumulated (a separate one-block launch, so the counts are complete
+// and visible without per-workgroup fences), src/evolve.cu:378-384.
+__global__ __launch_bounds__(kBlock) void k_batch_finish(const MCDevParams *__restrict__ params, MCDevState *states,
+                                                         int nrep) {
   for (int r = threadIdx.x; r < nrep; r += kBlock) {
     MCDevState *st = states + r;
-    const uint32_t acc = atomicAdd(&st->acc_count, 0u);
+    const uint32_t acc = st->acc_count;
     float a = st->ang_var;
     const double rate = 1.0 * (double)acc / (double)params[r].valid;
     if (rate < 0.5) {
@@ -190,7 +311,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
     st->acc_count = 0u;
     st->sweep = st->sweep + 1u;
   }
-  if (threadIdx.x == 0) *blocks_done = 0u;
 }
 
 }  // namespace
@@ -199,18 +319,16 @@ void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finali
   switch (B.potential) {
     case MCD_GHRL:
       k_batch_x4<MCD_GHRL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
-                                                          B.xcd, B.blocks_done);
+                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, B.xcd);
       break;
     case MCD_PEAR:
       k_batch_x4<MCD_PEAR><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
-                                                          B.xcd, B.blocks_done);
+                                                          B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, B.xcd);
       break;
     default:
       k_batch_x4<MCD_LL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                        B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, finalize,
-                                                        B.xcd, B.blocks_done);
+                                                        B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, B.xcd);
       break;
   }
+  if (finalize) k_batch_finish<<<1, kBlock, 0, B.stream>>>(B.params, B.states, B.nrep);
 }

@@ -103,17 +103,21 @@ def test_training_lowers_the_loss_and_raises_acceptance():
     assert acc1 > acc0 + 0.05, (acc0, acc1)
 
 
-def test_learned_sweep_decisions_are_exact_mh():
+@pytest.mark.parametrize("dims,bound", [((12, 12, 12), (1, 1, 1)), ((7, 6, 6), (0, 1, 1))])
+def test_learned_sweep_decisions_are_exact_mh(dims, bound):
     # Replays one recorded learned sweep: the oracle recomputes, in FP64, the
     # energy change and the proposal-density ratio from the recorded mu/kappa
     # and must reach the same decision for every move that is not a near tie.
-    L, T = 12, 1.1
+    # The 7 x 6 x 6 lattice with a free x axis has colour-set slots past the
+    # last column.
+    nx, ny, nz = dims
+    T = 1.1
     rng = np.random.default_rng(4)
     P = rand_params(rng, 0.1)
-    po = O.params(L, L, L)
-    pt = np.ones(L**3, np.int32)
-    with mclics.Network(P) as net, mclics.Lattice(mclics.Config(L, L, L, T=T, seed=5),
-                                                  mclics.random_ic(L, L, L, seed=5)) as lat:
+    po = O.params(nx, ny, nz, bound=bound)
+    pt = np.ones(nx * ny * nz, np.int32)
+    with mclics.Network(P) as net, mclics.Lattice(mclics.Config(nx, ny, nz, bound=bound, T=T, seed=5),
+                                                  mclics.random_ic(nx, ny, nz, seed=5)) as lat:
         lat.sweeps(50)
         start = lat.directors()
         lat.set_learned(net)
@@ -122,10 +126,11 @@ def test_learned_sweep_decisions_are_exact_mh():
     state = start.copy().reshape(-1, 3)
     near, checked = 0, 0
     for colour in (0, 1):
-        sites = [i + L * (j + L * k) for k in range(L) for j in range(L) for i in range(L) if (i + j + k) % 2 == colour]
+        sites = [i + nx * (j + ny * k) for k in range(nz) for j in range(ny) for i in range(nx)
+                 if (i + j + k) % 2 == colour]
         flat = state.reshape(-1).copy()
         for s in sites:
-            i, j, k = s % L, (s // L) % L, s // (L * L)
+            i, j, k = s % nx, (s // nx) % ny, s // (nx * ny)
             q = rec["prop"][3 * s:3 * s + 3]
             assert abs(np.linalg.norm(q.astype(np.float64)) - 1) < 1e-5
             import ctypes as C
