@@ -165,6 +165,13 @@ int mc_set_directors_aos(mc_handle *h, const float *ni_aos);
 
 /* Accepted count of the last completed sweep and the sweep counter. */
 int mc_get_acceptance(mc_handle *h, uint32_t *accepted, uint32_t *sweeps_done);
+/* Sets the sweep counter (the index of the next sweep, which keys the
+ * generator).  With the directors (mc_get/set_directors_aos), ang_var
+ * (mc_get/set_ang_var) and this counter a chain can be saved and resumed
+ * exactly: the generator has no other state.  Extends the reference's
+ * restart from a snapshot file (src/ic.cpp:88-119), which restores the
+ * directors only. */
+int mc_set_sweep_index(mc_handle *h, uint32_t sweep);
 
 /* Parity hook: one pass over a site set with injected proposals (AoS
  * float[3*Nt]) and uniforms (float[Nt]) at the current T; set 0..7 is a

@@ -48,6 +48,8 @@ __global__ void k_set_state_f(MCDevState *st, int field, float v) {
   if (field == 2) st->ang_var = v;
 }
 
+__global__ void k_set_sweep_index(MCDevState *st, uint32_t sweep) { st->sweep = sweep; }
+
 __global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var) {
   st->T = T;
   st->elecE = elecE;
@@ -544,6 +546,14 @@ int mc_get_ang_var(mc_handle *h, float *a) {
   MC_HIP(hipMemcpyAsync(&s, h->st, sizeof(s), hipMemcpyDeviceToHost, h->stream));
   MC_HIP(hipStreamSynchronize(h->stream));
   *a = s.ang_var;
+  return MC_OK;
+}
+
+int mc_set_sweep_index(mc_handle *h, uint32_t sweep) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  k_set_sweep_index<<<1, 1, 0, h->stream>>>(h->st, sweep);
+  MC_LAUNCHED("set sweep index");
   return MC_OK;
 }
 

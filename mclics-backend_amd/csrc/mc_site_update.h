@@ -121,14 +121,22 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
 }
 
 // FP32 screen of one Lebwohl-Lasher move: 1 accept, 0 reject, -1 decide
-// exactly.
+// exactly.  sum_b (q.m_b)^2 - (o.m_b)^2 = sum_b ((q-o).m_b) ((q+o).m_b).
+// Error (u = 2^-24, unit vectors, |q-o| |q+o| <= 2): each dot <= 4u|.|, each
+// bond product <= 16u, the running sum <= 72u, so <= 168u * 1.5|A| after the
+// scale; the reference's own float/double rounding of the two site energies
+// and their difference is <= 114u * 1.5|A|; with the final product, the
+// distance to the reference's dE stays below 300u * 1.5|A|, under a third of
+// eps = 2^-14 * 1.5|A| = 1024u * 1.5|A|.
 __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S) {
+  const float d0 = S.q[0] - S.o[0], d1 = S.q[1] - S.o[1], d2 = S.q[2] - S.o[2];
+  const float s0 = S.q[0] + S.o[0], s1 = S.q[1] + S.o[1], s2 = S.q[2] + S.o[2];
   float acc = 0.0f;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
-    const float a = fmaf(S.o[2], S.m[b][2], fmaf(S.o[1], S.m[b][1], S.o[0] * S.m[b][0]));
-    const float c = fmaf(S.q[2], S.m[b][2], fmaf(S.q[1], S.m[b][1], S.q[0] * S.m[b][0]));
-    acc = fmaf(c - a, c + a, acc);
+    const float e1 = fmaf(d2, S.m[b][2], fmaf(d1, S.m[b][1], d0 * S.m[b][0]));
+    const float e2 = fmaf(s2, S.m[b][2], fmaf(s1, S.m[b][1], s0 * S.m[b][0]));
+    acc = fmaf(e1, e2, acc);
   }
   float dE = k.cAf * acc;
   if (P.elecA != 0.0f)

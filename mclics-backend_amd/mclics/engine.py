@@ -56,6 +56,7 @@ EXPORTS = (
     "mc_get_directors_aos",
     "mc_set_directors_aos",
     "mc_get_acceptance",
+    "mc_set_sweep_index",
     "mc_pass",
     "mc_record_sweep",
     "mc_set_tuning",
@@ -167,6 +168,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_get_directors_aos.argtypes = [vp, f32p]
     lib.mc_set_directors_aos.argtypes = [vp, f32p]
     lib.mc_get_acceptance.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.mc_set_sweep_index.argtypes = [vp, C.c_uint32]
     lib.mc_pass.argtypes = [vp, C.c_int, f32p, f32p, u8p, f32p, f32p]
     lib.mc_record_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p, f32p]
     ip = C.POINTER(C.c_int)
@@ -333,6 +335,15 @@ class Lattice:
         a, s = C.c_uint32(), C.c_uint32()
         _check(self._lib.mc_get_acceptance(self._h, C.byref(a), C.byref(s)))
         return a.value, s.value
+
+    def set_sweep_index(self, sweep: int) -> None:
+        """Index of the next sweep (keys the generator); with directors and
+        ang_var it resumes a saved chain exactly."""
+        _check(self._lib.mc_set_sweep_index(self._h, sweep))
+
+    def checkpoint(self):
+        """(directors, ang_var, sweep index): everything the chain's future depends on."""
+        return self.directors(), self.ang_var(), self.acceptance()[1]
 
     def local_order(self) -> np.ndarray:
         """Local S of every site (NaN where pt == 0), as in the snapshot files."""

@@ -429,3 +429,34 @@ def test_slab_measure_series_reads_the_current_planes():
             b.sweeps(1)
             e, s_, _ = b.measure()
             assert E[w] == e and S[w] == s_, w
+
+
+@pytest.mark.parametrize("kernel", ["slab", "generic", "ref8"])
+def test_checkpoint_resumes_the_chain_exactly(kernel):
+    # 12 sweeps in one go equal 5 sweeps, a checkpoint (directors, ang_var,
+    # sweep index), a fresh handle restored from it and 7 more sweeps.
+    nx, ny, nz = 16, 8, 12
+    n0 = unit(np.random.default_rng(23), nx * ny * nz)
+    order = mclics.ORDER_REF8 if kernel == "ref8" else mclics.ORDER_CHECKER
+    cfg = mclics.Config(nx, ny, nz, T=1.05, seed=77, order=order)
+
+    def make(n):
+        lat = mclics.Lattice(cfg, n)
+        if kernel == "generic":
+            lat.set_kernel(mclics.KERNEL_GENERIC)
+        return lat
+
+    with make(n0) as a:
+        a.sweeps(12, graph=True)
+        want = a.checkpoint()
+    with make(n0) as b:
+        b.sweeps(5)
+        d, av, sw = b.checkpoint()
+    assert sw == 5
+    with make(d) as c:
+        c.set_ang_var(av)
+        c.set_sweep_index(sw)
+        c.sweeps(7, graph=True)
+        got = c.checkpoint()
+    assert np.array_equal(got[0], want[0])
+    assert got[1:] == want[1:]
