@@ -238,9 +238,7 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
   const int per = passes_per_sweep(h);
   const int finalize = t == per - 1;
   if (learned_sweep(h)) {
-    const int slots = mcd_set_slots(h->P, 8 + t);
-    int grid = (slots + 63) / 64;
-    if (grid > h->cu_count * 16) grid = h->cu_count * 16;
+    const int grid = mc_mlp_learned_grid(mcd_set_slots(h->P, 8 + t), h->cu_count);
     mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
   } else if (use_slab(h)) {
     mc_launch_sweep_slab(L, h->bx, h->by, h->bz, slab_shape(h), rec);

@@ -73,7 +73,7 @@ namespace {
 void free_mlp(mc_mlp *m) {
   if (!m) return;
   void *ptrs[] = {m->B.params, m->B.grad, m->B.m1, m->B.m2, m->B.loss, m->B.W1b, m->B.W2b, m->B.W3b, m->B.W3Tb,
-                  m->B.W2Tb, m->B.b1, m->B.b2, m->B.b3, m->X, m->base, m->dirs, m->tmp};
+                  m->B.W2Tb, m->B.b1, m->B.b2, m->B.b3, m->B.ops, m->X, m->base, m->dirs, m->tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -161,6 +161,7 @@ int mc_mlp_create(const float *params, uint64_t seed, mc_mlp **out) {
   ML_HIPC(hipMalloc(&m->B.b1, 64 * sizeof(float)));
   ML_HIPC(hipMalloc(&m->B.b2, 64 * sizeof(float)));
   ML_HIPC(hipMalloc(&m->B.b3, 4 * sizeof(float)));
+  ML_HIPC(hipMalloc(&m->B.ops, MC_MLP_OPS_BYTES));
   std::vector<float> P;
   if (params)
     P.assign(params, params + MC_MLP_NPARAM);

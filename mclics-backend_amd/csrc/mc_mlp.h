@@ -14,7 +14,13 @@ struct MCMLPBuffers {
   double *loss;   // loss sum
   __bf16 *W1b, *W2b, *W3b, *W3Tb, *W2Tb;  // bf16 operand layouts
   float *b1, *b2, *b3;
+  // The forward pass's 14 weight operands lane by lane (bf16, 1 KiB each),
+  // then b1, b2, b3 (FP32, 144 floats): the block the learned pass stages
+  // into LDS with plain 16-byte copies.
+  void *ops;
 };
+
+#define MC_MLP_OPS_BYTES (14 * 1024 + 144 * 4)
 
 // Per-site capture of one learned pass (device arrays of Nt; all null = off).
 struct MCLearnRecord {
@@ -34,6 +40,9 @@ void mc_mlp_launch_grad(const MCMLPBuffers &B, hipStream_t s, const float *X, co
 void mc_mlp_launch_adam(const MCMLPBuffers &B, hipStream_t s, int nsamp, float lr, int step);
 void mc_mlp_launch_fill(float *p, int n, float v, hipStream_t s);
 void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, float *base, float *dirs);
+// Workgroups of a learned pass over `slots` sites: one 64-site block per
+// wave, up to the five workgroups per CU that fit in LDS.
+int mc_mlp_learned_grid(int slots, int cu_count);
 void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
                                 const MCLearnRecord *rec);
 

@@ -127,6 +127,50 @@ __device__ void forward64t(const MLPDev &N, const __bf16 *Xs, float *Os) {
   __syncthreads();
 }
 
+// The same forward with the weight operands and biases in LDS (the block
+// k_mlp_pack_ops writes, staged once per workgroup): operand b of lane l is
+// the 16 bytes at Wl + 1024 b + 16 l, so every weight read is one
+// conflict-free ds_read_b128 from a single per-lane address, and the biases
+// are broadcast reads.  The input rows are 24 bf16 wide (Xs, 64 x 24): K
+// values 24..31 are zero padding, so lanes with l/16 == 3 take a zero
+// operand instead of reading them.  Xs and Os belong to the calling wave
+// alone, which orders its own LDS writes and reads (wave_sync).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void forward64l(const __bf16 *Wl, const float *Bl, const __bf16 *Xs, float *Os) {
+  const int l = threadIdx.x & 63;
+  const int r = l & 15, g = l >> 4;
+  const bf16x8 *W = reinterpret_cast<const bf16x8 *>(Wl) + l;  // operand b at W[64 b]
+#pragma unroll 1
+  for (int t = 0; t < 4; ++t) {
+    // Compiler-only barrier: the operands are read again for every tile
+    // instead of being hoisted out of the loop into 88 long-lived registers.
+    asm volatile("" ::: "memory");
+    const bf16x8 x = g < 3 ? load8(Xs + (16 * t + r) * 24 + 8 * g) : bf16x8{};
+    f4 h1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+      h1[mt] = mfma(W[64 * mt], x, *reinterpret_cast<const f4 *>(Bl + 16 * mt + 4 * g));
+    const bf16x8 a0 = pack_hidden(h1[0], h1[1]), a1 = pack_hidden(h1[2], h1[3]);
+    f4 h2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f4 c = mfma(W[64 * (4 + 2 * mt)], a0, *reinterpret_cast<const f4 *>(Bl + 64 + 16 * mt + 4 * g));
+      h2[mt] = mfma(W[64 * (5 + 2 * mt)], a1, c);
+    }
+    const bf16x8 b0 = pack_hidden(h2[0], h2[1]), b1 = pack_hidden(h2[2], h2[3]);
+    f4 o = g == 0 ? *reinterpret_cast<const f4 *>(Bl + 128) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+    o = mfma(W[64 * 12], b0, o);
+    o = mfma(W[64 * 13], b1, o);
+    if (g == 0) *reinterpret_cast<f4 *>(Os + (16 * t + r) * 4) = o;
+  }
+  wave_sync();
+}
+
 __device__ __forceinline__ void load_weights(const MLPDev &N, bf16x8 w1[4], bf16x8 w2[4][2], bf16x8 w3[2]) {
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
@@ -185,6 +229,20 @@ __device__ __forceinline__ void write_xrow(__bf16 *Xs, int row, const float x[22
   }
 }
 
+// Row `row` of the 24-wide input rows of forward64l (features 22, 23 zero).
+__device__ __forceinline__ void write_xrow24(__bf16 *Xs, int row, const float x[22], bool valid) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * c + e;
+      v[e] = (__bf16)((valid && k < 22) ? x[k] : 0.0f);
+    }
+    *reinterpret_cast<bf16x8 *>(Xs + row * 24 + 8 * c) = v;
+  }
+}
+
 // log cosh(z) + log 2, FP32.
 // log(1 + y) for y = e^{-2|z|} in (0, 1] with the hardware log: 1 + y is
 // rounded once (absolute error <= 2^-24 in the result), where the library
@@ -220,6 +278,37 @@ __global__ void k_mlp_pack(const float *__restrict__ P, __bf16 *W1b, __bf16 *W2b
       b2[t] = P[B2_OFF + t];
     }
     if (t < 4) b3[t] = P[B3_OFF + t];
+  }
+}
+
+// The operand block of forward64l from the bf16 layouts: operands 0..3 are
+// W1 rows 16 mt + l%16 (K 8(l/16)..+7), 4 + 2 mt + c the W2 chunks of
+// aop_hidden, 12 + c the W3 chunks; then b1, b2, b3 and 12 zero floats.
+__global__ void k_mlp_pack_ops(const __bf16 *__restrict__ W1b, const __bf16 *__restrict__ W2b,
+                               const __bf16 *__restrict__ W3b, const float *__restrict__ b1,
+                               const float *__restrict__ b2, const float *__restrict__ b3, void *ops) {
+  __bf16 *W = reinterpret_cast<__bf16 *>(ops);
+  float *B = reinterpret_cast<float *>(reinterpret_cast<char *>(ops) + 14 * 1024);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 14 * 64 + 144; t += gridDim.x * blockDim.x) {
+    if (t >= 14 * 64) {
+      const int k = t - 14 * 64;
+      B[k] = k < 64 ? b1[k] : (k < 128 ? b2[k - 64] : (k < 132 ? b3[k - 128] : 0.0f));
+      continue;
+    }
+    const int b = t >> 6, l = t & 63, r = l & 15, g = l >> 4;
+    bf16x8 v;
+    if (b < 4) {
+      v = load8(W1b + (16 * b + r) * 32 + 8 * g);
+    } else {
+      const __bf16 *M = b < 12 ? W2b : W3b;
+      const int row = b < 12 ? 16 * ((b - 4) >> 1) + r : r, c = b < 12 ? (b - 4) & 1 : b - 12;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = M[row * 64 + 32 * c + 4 * g + e];
+        v[4 + e] = M[row * 64 + 32 * c + 16 + 4 * g + e];
+      }
+    }
+    *reinterpret_cast<bf16x8 *>(W + 8 * t) = v;
   }
 }
 
@@ -586,22 +675,37 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
 }
 
 // One colour pass of the learned-proposal Metropolis-Hastings sweep over an
-// all-active lattice; one wave per workgroup, 64 sites per iteration.
-template <int POT>
-__global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, float *__restrict__ nx,
-                                                     float *__restrict__ ny, float *__restrict__ nz, MCDevState *st,
-                                                     int colour, int finalize, MCLearnRecord rec) {
-  __shared__ __align__(16) __bf16 Xs[64 * 32];
-  __shared__ __align__(16) float Os[64 * 4];
+// all-active lattice.  A workgroup of kLearnWaves waves stages the network
+// (weight operands and biases, 14.6 KiB) into LDS once; then each wave takes
+// 64-site blocks on its own (blocks wave, wave + kLearnWaves * gridDim.x,
+// ...) with its own input rows and outputs in LDS, so the only workgroup
+// barriers are after staging and at the final count.
+constexpr int kLearnWaves = 4;
+
+template <int POT, bool RECORD>
+__global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_eu(5))) void k_learned_pass(
+    MCDevParams P, const uint4 *__restrict__ ops, float *__restrict__ nx, float *__restrict__ ny,
+    float *__restrict__ nz, MCDevState *st, int colour, int finalize, MCLearnRecord rec) {
+  __shared__ __align__(16) uint4 Wst[MC_MLP_OPS_BYTES / 16];
+  __shared__ __align__(16) __bf16 Xall[kLearnWaves][64 * 24];
+  __shared__ __align__(16) float Oall[kLearnWaves][64 * 4];
+  __shared__ uint32_t wsum[kLearnWaves];
+  for (int i = threadIdx.x; i < MC_MLP_OPS_BYTES / 16; i += 64 * kLearnWaves) Wst[i] = ops[i];
+  __syncthreads();
+  const __bf16 *Wl = reinterpret_cast<const __bf16 *>(Wst);
+  const float *Bl = reinterpret_cast<const float *>(Wst + 14 * 64);
+  const int wave = threadIdx.x >> 6;
+  __bf16 *Xs = Xall[wave];
+  float *Os = Oall[wave];
 
   const float T = st->T, E = st->elecE;
   const uint32_t sweep = st->sweep;
   const double cA = (double)(-P.A) * 1.5;
   const float cAf = (float)cA;
-  const int l = threadIdx.x;
+  const int l = threadIdx.x & 63;
   const int total = mcd_set_slots(P, 8 + colour);
   uint32_t acc = 0;
-  for (int blk = blockIdx.x; blk * 64 < total; blk += gridDim.x) {
+  for (int blk = blockIdx.x * kLearnWaves + wave; blk * 64 < total; blk += gridDim.x * kLearnWaves) {
     const int t = blk * 64 + l;
     int i = 0, j = 0, k = 0;
     bool valid = t < total && mcd_set_site(P, 8 + colour, t, i, j, k);
@@ -614,10 +718,10 @@ __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, fl
       gather6(P, nx, ny, nz, i, j, k, m, on);
       float x[22];
       features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
-      write_xrow(Xs, l, x, valid);
+      write_xrow24(Xs, l, x, valid);
     }
-    __syncthreads();
-    forward64t(N, Xs, Os);
+    wave_sync();
+    forward64l(Wl, Bl, Xs, Os);
     float mu[3], lk;
     head(Os + l * 4, b, mu, lk);
     // The neighbours again (cache hits) rather than 18 registers held live
@@ -662,7 +766,7 @@ __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, fl
       nz[s] = q[2];
     }
     acc += a ? 1u : 0u;
-    if (rec.mu && valid) {
+    if (RECORD && valid) {
       rec.mu[3 * (size_t)s] = mu[0];
       rec.mu[3 * (size_t)s + 1] = mu[1];
       rec.mu[3 * (size_t)s + 2] = mu[2];
@@ -674,15 +778,20 @@ __global__ __launch_bounds__(64) void k_learned_pass(MCDevParams P, MLPDev N, fl
       rec.acc[s] = a ? 1 : 0;
       rec.dE[s] = dE;
     }
-    __syncthreads();
+    wave_sync();  // this wave's Os reads before the next block's Xs/Os writes
   }
   // acceptance bookkeeping; ang_var (unused by the learned proposal) is left
   // as it is for plain sweeps interleaved with learned ones
   uint32_t v = acc;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (l == 0) wsum[wave] = v;
+  __syncthreads();
   // (one 64-bit atomic on the adjacent acc_count/blocks_done words in the
   // final pass, as mcd_finish_sweep: no fence needed)
   if (threadIdx.x == 0) {
+    v = 0;
+#pragma unroll
+    for (int w = 0; w < kLearnWaves; ++w) v += wsum[w];
     if (!finalize) {
       atomicAdd(&st->acc_count, v);
     } else {
@@ -714,6 +823,7 @@ MLPDev dev_of(const MCMLPBuffers &B) {
 
 void mc_mlp_launch_pack(const MCMLPBuffers &B, hipStream_t s) {
   k_mlp_pack<<<16, 256, 0, s>>>(B.params, B.W1b, B.W2b, B.W3b, B.W3Tb, B.W2Tb, B.b1, B.b2, B.b3);
+  k_mlp_pack_ops<<<4, 256, 0, s>>>(B.W1b, B.W2b, B.W3b, B.b1, B.b2, B.b3, B.ops);
 }
 
 void mc_mlp_launch_infer(const MCMLPBuffers &B, hipStream_t s, const float *X, const float *base, int n, float *mu,
@@ -741,19 +851,34 @@ void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, fl
   k_mlp_dataset<<<grid, 256, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, n, seed, X, base, dirs);
 }
 
+int mc_mlp_learned_grid(int slots, int cu_count) {
+  const int blocks = (slots + 63) / 64;
+  int grid = (blocks + kLearnWaves - 1) / kLearnWaves;
+  if (grid > cu_count * 5) grid = cu_count * 5;
+  return grid > 0 ? grid : 1;
+}
+
 void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
                                 const MCLearnRecord *rec) {
   const MCLearnRecord none{};
   const MCLearnRecord &R = rec ? *rec : none;
+  const uint4 *ops = reinterpret_cast<const uint4 *>(B.ops);
+  const int threads = 64 * kLearnWaves;
+#define MC_LEARNED(POT_)                                                                                           \
+  if (rec)                                                                                                         \
+    k_learned_pass<POT_, true><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R); \
+  else                                                                                                             \
+    k_learned_pass<POT_, false><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R)
   switch (L.P.potential) {
     case MCD_GHRL:
-      k_learned_pass<MCD_GHRL><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      MC_LEARNED(MCD_GHRL);
       break;
     case MCD_PEAR:
-      k_learned_pass<MCD_PEAR><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      MC_LEARNED(MCD_PEAR);
       break;
     default:
-      k_learned_pass<MCD_LL><<<grid, 64, 0, L.stream>>>(L.P, dev_of(B), L.nx, L.ny, L.nz, L.st, colour, finalize, R);
+      MC_LEARNED(MCD_LL);
       break;
   }
+#undef MC_LEARNED
 }
