@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -50,7 +51,9 @@ __global__ void k_set_state_f(MCDevState *st, int field, float v) {
 
 __global__ void k_set_sweep_index(MCDevState *st, uint32_t sweep) { st->sweep = sweep; }
 
-__global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var) {
+__global__ void k_init_state(MCDevState *st, float T, float elecE, float ang_var, unsigned long long *ctr) {
+  for (int c = threadIdx.x; c < MCD_NSUB; c += blockDim.x) ctr[(size_t)c * MCD_SUB_STRIDE] = 0ull;
+  if (threadIdx.x != 0) return;
   st->T = T;
   st->elecE = elecE;
   st->ang_var = ang_var;
@@ -90,6 +93,7 @@ struct mc_handle {
   uint8_t *pt = nullptr;
   float *sx = nullptr, *sy = nullptr, *sz = nullptr;
   MCDevState *st = nullptr;
+  unsigned long long *ctr = nullptr;  // sweep sub-counters (P.ctr)
   double *partials = nullptr;
   double *result = nullptr;
   double *series = nullptr;  // per-step {E, S} of mc_measure_series
@@ -131,7 +135,7 @@ void free_handle(mc_handle *h) {
   if (!h) return;
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
-  void *ptrs[] = {h->nx, h->ny, h->nz, h->bx, h->by, h->bz, h->pt, h->sx, h->sy, h->sz, h->st, h->partials,
+  void *ptrs[] = {h->nx, h->ny, h->nz, h->bx, h->by, h->bz, h->pt, h->sx, h->sy, h->sz, h->st, h->ctr, h->partials,
                   h->result, h->staging, h->pbuf, h->abuf, h->series};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -238,7 +242,9 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
   const int per = passes_per_sweep(h);
   const int finalize = t == per - 1;
   if (learned_sweep(h)) {
-    const int grid = mc_mlp_learned_grid(mcd_set_slots(h->P, 8 + t), h->cu_count);
+    // one grid for both colours (the sweep counters need it)
+    const int slots = std::max(mcd_set_slots(h->P, 8), mcd_set_slots(h->P, 9));
+    const int grid = mc_mlp_learned_grid(slots, h->cu_count);
     mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
   } else if (use_slab(h)) {
     mc_launch_sweep_slab(L, h->bx, h->by, h->bz, slab_shape(h), rec);
@@ -248,7 +254,9 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
     mc_launch_sweep_x4(L, t, g, finalize, h->x4_xcd, rec);
   } else {
     const int set = per == 8 ? t : 8 + t;
-    mc_launch_sweep_pass(L, set, h->grid_for(mc_set_slots(h->P, set)), finalize, rec);
+    int slots = 0;  // one grid for every pass of the sweep (the sweep counters need it)
+    for (int u = 0; u < per; ++u) slots = std::max(slots, mc_set_slots(h->P, per == 8 ? u : 8 + u));
+    mc_launch_sweep_pass(L, set, h->grid_for(slots), finalize, rec);
   }
 }
 
@@ -471,6 +479,8 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   MC_HIPC(hipMalloc(&h->nz, fbytes));
   MC_HIPC(hipMalloc(&h->staging, 3 * fbytes));
   MC_HIPC(hipMalloc(&h->st, sizeof(MCDevState)));
+  MC_HIPC(hipMalloc(&h->ctr, (size_t)MCD_NSUB * MCD_SUB_STRIDE * sizeof(unsigned long long)));
+  h->P.ctr = h->ctr;
   h->measure_grid = h->grid_for(nt);
   MC_HIPC(hipMalloc(&h->partials, (size_t)h->measure_grid * 8 * sizeof(double)));
   MC_HIPC(hipMalloc(&h->result, 16 * sizeof(double)));
@@ -497,7 +507,7 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
     MC_HIPC(e1);
     MC_HIPC(e2);
   }
-  k_init_state<<<1, 1, 0, h->stream>>>(h->st, p->T, p->elecE, p->ang_var);
+  k_init_state<<<1, 64, 0, h->stream>>>(h->st, p->T, p->elecE, p->ang_var, h->ctr);
   MC_HIPC(hipGetLastError());
   MC_HIPC(hipStreamSynchronize(h->stream));
   rc = select_kernel(h, MC_KERNEL_AUTO);

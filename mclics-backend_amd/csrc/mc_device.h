@@ -69,7 +69,16 @@ struct MCDevParams {
   int valid;
   // dividers of the site-set maps: (nx+1)/2, ny, Ni = (nx+1)/2, Nj = (ny+1)/2, Ni*Nj
   MCDiv div_hx, div_ny, div_nj, div_ninj;
+  // MCD_NSUB sweep sub-counters, MCD_SUB_STRIDE words apart (mcd_finish_sweep)
+  unsigned long long *ctr;
 };
+
+// Sub-counters of the sweep bookkeeping: same-address device atomics
+// serialise (about 11.5 ns each on MI355X, so 2048 workgroups counting into
+// one word cost ~24 us per pass), while atomics on different 128-byte lines
+// proceed in parallel.
+#define MCD_NSUB 64
+#define MCD_SUB_STRIDE 16
 
 // Mutable sweep state in device memory, so captured graphs replay with the
 // current temperature, field and proposal width.
@@ -330,33 +339,50 @@ __device__ __forceinline__ uint32_t mcd_block_sum(uint32_t v) {
 // Device-side ang_var adaptation by the last workgroup of a sweep's final
 // pass, src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
 //
-// acc_count (low word) and blocks_done (high word) are adjacent in
-// MCDevState, so a final-pass workgroup adds (1 << 32) | count in ONE 64-bit
-// atomic: the value it gets back tells it whether it is the last one and
-// already holds every other workgroup's count, so no memory fence is needed
-// (an agent-scope fence in every workgroup costs an L2 write-back each).
-// Earlier passes of the sweep add their counts to the low word; they ended
-// with their kernel, so the final pass's atomics see them.
-__device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, uint32_t block_acc, bool finalize) {
+// End of a workgroup's part of a sweep pass, called by every thread after the
+// block sum (thread 0 holds block_acc).  Every pass of one sweep must use the
+// same grid G.  With S = min(G, MCD_NSUB), workgroup b counts into sub-counter
+// b % S: its accepted moves in the low word and, in the final pass, one
+// arrival in the high word, as one 64-bit atomic.  The last arrival of a sub
+// (its arrival count is known from G and S) clears it and moves the sub's
+// total into the state's acc_count/blocks_done word; the last of the S subs
+// to arrive there holds the sweep's count, adapts ang_var (when asked,
+// src/evolve.cu:378-384), clears the word and advances the sweep index.
+// Each sub sees at most ceil(G/S) atomics and the state word S.  The 64-bit
+// atomics order each count with its arrival, so no fence is needed (an
+// agent-scope fence in every workgroup costs an L2 write-back each); earlier
+// passes of the sweep ended with their kernels, so their counts are seen.
+__device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, uint32_t block_acc, bool finalize,
+                                        bool adapt = true) {
   if (threadIdx.x != 0) return;
+  const uint32_t G = gridDim.x, S = G < MCD_NSUB ? G : MCD_NSUB;
+  const uint32_t c = blockIdx.x % S;
+  unsigned long long *sub = P.ctr + (size_t)c * MCD_SUB_STRIDE;
   if (!finalize) {
-    atomicAdd(&st->acc_count, block_acc);
+    atomicAdd(sub, (unsigned long long)block_acc);
     return;
   }
+  const uint32_t arrivals = G / S + (c < G % S ? 1u : 0u);
+  const unsigned long long old = atomicAdd(sub, (1ull << 32) | (unsigned long long)block_acc);
+  if ((uint32_t)(old >> 32) != arrivals - 1u) return;
+  *sub = 0ull;
+  const uint32_t part = (uint32_t)old + block_acc;
   unsigned long long *word = reinterpret_cast<unsigned long long *>(&st->acc_count);
-  const unsigned long long old = atomicAdd(word, (1ull << 32) | (unsigned long long)block_acc);
-  if ((uint32_t)(old >> 32) != gridDim.x - 1) return;
-  const uint32_t acc = (uint32_t)old + block_acc;
-  float a = st->ang_var;
-  const double rate = 1.0 * (double)acc / (double)P.valid;
-  if (rate < 0.5) {
-    a = (float)((double)a * 0.99);
-    if ((double)a < 0.01) a = (float)0.01;
-  } else if (rate > 0.5) {
-    a = (float)((double)a / 0.99);
-    if ((double)a > 1.0) a = (float)((double)a - 0.5);
+  const unsigned long long old2 = atomicAdd(word, (1ull << 32) | (unsigned long long)part);
+  if ((uint32_t)(old2 >> 32) != S - 1u) return;
+  const uint32_t acc = (uint32_t)old2 + part;
+  if (adapt) {
+    float a = st->ang_var;
+    const double rate = 1.0 * (double)acc / (double)P.valid;
+    if (rate < 0.5) {
+      a = (float)((double)a * 0.99);
+      if ((double)a < 0.01) a = (float)0.01;
+    } else if (rate > 0.5) {
+      a = (float)((double)a / 0.99);
+      if ((double)a > 1.0) a = (float)((double)a - 0.5);
+    }
+    st->ang_var = a;
   }
-  st->ang_var = a;
   st->last_acc = acc;
   *word = 0ull;  // acc_count = blocks_done = 0 for the next sweep
   st->sweep = st->sweep + 1u;

@@ -786,24 +786,12 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if (l == 0) wsum[wave] = v;
   __syncthreads();
-  // (one 64-bit atomic on the adjacent acc_count/blocks_done words in the
-  // final pass, as mcd_finish_sweep: no fence needed)
   if (threadIdx.x == 0) {
     v = 0;
 #pragma unroll
     for (int w = 0; w < kLearnWaves; ++w) v += wsum[w];
-    if (!finalize) {
-      atomicAdd(&st->acc_count, v);
-    } else {
-      unsigned long long *word = reinterpret_cast<unsigned long long *>(&st->acc_count);
-      const unsigned long long old = atomicAdd(word, (1ull << 32) | (unsigned long long)v);
-      if ((uint32_t)(old >> 32) == gridDim.x - 1) {
-        st->last_acc = (uint32_t)old + v;
-        *word = 0ull;
-        st->sweep = st->sweep + 1u;
-      }
-    }
   }
+  mcd_finish_sweep(P, st, v, finalize != 0, /*adapt=*/false);
 }
 
 MLPDev dev_of(const MCMLPBuffers &B) {
