@@ -244,7 +244,7 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
   if (learned_sweep(h)) {
     // one grid for both colours (the sweep counters need it)
     const int slots = std::max(mcd_set_slots(h->P, 8), mcd_set_slots(h->P, 9));
-    const int grid = mc_mlp_learned_grid(slots, h->cu_count);
+    const int grid = mc_mlp_learned_grid(h->P, slots, h->cu_count);
     mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
   } else if (use_slab(h)) {
     mc_launch_sweep_slab(L, h->bx, h->by, h->bz, slab_shape(h), rec);

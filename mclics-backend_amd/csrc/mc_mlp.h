@@ -41,8 +41,8 @@ void mc_mlp_launch_adam(const MCMLPBuffers &B, hipStream_t s, int nsamp, float l
 void mc_mlp_launch_fill(float *p, int n, float v, hipStream_t s);
 void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, float *base, float *dirs);
 // Workgroups of a learned pass over `slots` sites: one 64-site block per
-// wave, up to the five workgroups per CU that fit in LDS.
-int mc_mlp_learned_grid(int slots, int cu_count);
+// wave, up to the workgroups every CU holds at once.
+int mc_mlp_learned_grid(const MCDevParams &P, int slots, int cu_count);
 void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
                                 const MCLearnRecord *rec);
 

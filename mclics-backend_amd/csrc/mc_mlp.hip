@@ -85,14 +85,20 @@ __device__ __forceinline__ bf16x8 aop_hidden(const __bf16 *W, int ld, int row, i
 }
 
 // ReLU, bf16 rounding and the D-tile to B-operand packing of K chunk c.
+// Rounding first and then taking the ReLU as a signed 16-bit max with 0 on
+// the bf16 bit patterns (v_pk_max_i16, two values per instruction) gives the
+// same bits as rounding the ReLU: a negative input rounds to a value <= 0,
+// whose pattern (sign bit set, or +0) maps to +0.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ bf16x8 pack_hidden(const f4 &a, const f4 &b) {
   bf16x8 v;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    v[e] = (__bf16)fmaxf(a[e], 0.0f);
-    v[4 + e] = (__bf16)fmaxf(b[e], 0.0f);
+    v[e] = (__bf16)a[e];
+    v[4 + e] = (__bf16)b[e];
   }
-  return v;
+  const s16x8 w = __builtin_elementwise_max(__builtin_bit_cast(s16x8, v), s16x8{0, 0, 0, 0, 0, 0, 0, 0});
+  return __builtin_bit_cast(bf16x8, w);
 }
 
 // The weight operands (rows 16 mt + l%16 of W1/W2, rows l%16 of W3; 14 KiB in
@@ -595,9 +601,26 @@ __global__ void k_fill_f32(float *p, int n, float v) {
 
 // Gathers the six neighbours of site (i,j,k) of an all-active lattice with
 // the lattice's boundary types (bulk neighbourhoods only).
+// PER: every axis periodic, so every neighbour exists and the six indices
+// are the site's own index plus or minus a stride, wrapped by one select.
+template <bool PER = false>
 __device__ __forceinline__ void gather6(const MCDevParams &P, const float *nx, const float *ny, const float *nz,
                                         int i, int j, int k, float m[6][3], bool on[6]) {
   const int X = P.nx, Y = P.ny, Z = P.nz;
+  if (PER) {
+    const int XY = X * Y, s = i + X * (j + Y * k);
+    const int idx[6] = {i + 1 == X ? s + 1 - X : s + 1,     i == 0 ? s - 1 + X : s - 1,
+                        j + 1 == Y ? s + X - XY : s + X,    j == 0 ? s - X + XY : s - X,
+                        k + 1 == Z ? s + XY - P.nt : s + XY, k == 0 ? s - XY + P.nt : s - XY};
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      on[b] = true;
+      m[b][0] = nx[idx[b]];
+      m[b][1] = ny[idx[b]];
+      m[b][2] = nz[idx[b]];
+    }
+    return;
+  }
   int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
   on[0] = on[1] = on[2] = on[3] = on[4] = on[5] = true;
   if (ip == X) { if (P.per[0]) ip = 0; else on[0] = false; }
@@ -682,8 +705,8 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
 // barriers are after staging and at the final count.
 constexpr int kLearnWaves = 4;
 
-template <int POT, bool RECORD>
-__global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_eu(5))) void k_learned_pass(
+template <int POT, bool RECORD, bool PER>
+__global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned_pass(
     MCDevParams P, const uint4 *__restrict__ ops, float *__restrict__ nx, float *__restrict__ ny,
     float *__restrict__ nz, MCDevState *st, int colour, int finalize, MCLearnRecord rec) {
   __shared__ __align__(16) uint4 Wst[MC_MLP_OPS_BYTES / 16];
@@ -715,7 +738,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     bool on[6];
     float b[3];
     {
-      gather6(P, nx, ny, nz, i, j, k, m, on);
+      gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
       float x[22];
       features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
       write_xrow24(Xs, l, x, valid);
@@ -726,7 +749,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     head(Os + l * 4, b, mu, lk);
     // The neighbours again (cache hits) rather than 18 registers held live
     // across the network.
-    gather6(P, nx, ny, nz, i, j, k, m, on);
+    gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
     const float kappa = __expf(lk);
     const MCU4 r = mcd_philox(MCU4{(uint32_t)s, sweep, P.stream, 0x4D4Cu}, P.seed_lo, P.seed_hi);
     float q[3];
@@ -839,10 +862,40 @@ void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, fl
   k_mlp_dataset<<<grid, 256, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, n, seed, X, base, dirs);
 }
 
-int mc_mlp_learned_grid(int slots, int cu_count) {
+// Workgroups of the pass's kernel that one CU holds at once (registers and
+// LDS), asked of the runtime once per instantiation.
+template <int POT, bool PER>
+int learned_resident() {
+  static int n = 0;
+  if (n == 0) {
+    int r = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, reinterpret_cast<const void *>(&k_learned_pass<POT, false, PER>),
+                                                     64 * kLearnWaves, 0) != hipSuccess || r < 1) {
+      (void)hipGetLastError();
+      r = 4;
+    }
+    n = r;
+  }
+  return n;
+}
+
+int mc_mlp_learned_grid(const MCDevParams &P, int slots, int cu_count) {
+  const bool per = P.per[0] && P.per[1] && P.per[2];
+  int res;
+  switch (P.potential) {
+    case MCD_GHRL:
+      res = per ? learned_resident<MCD_GHRL, true>() : learned_resident<MCD_GHRL, false>();
+      break;
+    case MCD_PEAR:
+      res = per ? learned_resident<MCD_PEAR, true>() : learned_resident<MCD_PEAR, false>();
+      break;
+    default:
+      res = per ? learned_resident<MCD_LL, true>() : learned_resident<MCD_LL, false>();
+      break;
+  }
   const int blocks = (slots + 63) / 64;
   int grid = (blocks + kLearnWaves - 1) / kLearnWaves;
-  if (grid > cu_count * 5) grid = cu_count * 5;
+  if (grid > cu_count * res) grid = cu_count * res;
   return grid > 0 ? grid : 1;
 }
 
@@ -852,11 +905,17 @@ void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int co
   const MCLearnRecord &R = rec ? *rec : none;
   const uint4 *ops = reinterpret_cast<const uint4 *>(B.ops);
   const int threads = 64 * kLearnWaves;
-#define MC_LEARNED(POT_)                                                                                           \
-  if (rec)                                                                                                         \
-    k_learned_pass<POT_, true><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R); \
-  else                                                                                                             \
-    k_learned_pass<POT_, false><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R)
+  const bool per = L.P.per[0] && L.P.per[1] && L.P.per[2];
+#define MC_LEARNED_(POT_, REC_, PER_) \
+  k_learned_pass<POT_, REC_, PER_><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R)
+#define MC_LEARNED(POT_)                  \
+  if (rec) {                              \
+    if (per) MC_LEARNED_(POT_, true, true);   \
+    else MC_LEARNED_(POT_, true, false);      \
+  } else {                                \
+    if (per) MC_LEARNED_(POT_, false, true);  \
+    else MC_LEARNED_(POT_, false, false);     \
+  }
   switch (L.P.potential) {
     case MCD_GHRL:
       MC_LEARNED(MCD_GHRL);
@@ -869,4 +928,5 @@ void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int co
       break;
   }
 #undef MC_LEARNED
+#undef MC_LEARNED_
 }
