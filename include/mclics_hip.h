@@ -192,10 +192,11 @@ int mc_record_sweep(mc_handle *h, float *u0, float *u1, float *u2, float *prop_a
  * Batched replicas.  nrep independent bulk lattices (periodic on every axis,
  * Nx % 4 == 0, even Ny and Nz, one potential for all) of any sizes, each with
  * its own parameters, temperature, seed/stream and proposal width, held in
- * one device allocation.  A sweep of all replicas is two persistent launches
- * (one per colour) over the tiles of every replica, replayed as one captured
- * graph; every replica evolves bit-identically to the same lattice run alone
- * with the segment kernel.  No reference counterpart: the reference runs one
+ * one device allocation.  A sweep of all replicas is one slab launch over the
+ * tiles of every replica (or two segment-kernel launches, one per colour),
+ * then a one-block finisher, replayed as one captured graph; every replica
+ * evolves bit-identically to the same lattice run alone with the segment
+ * kernel.  No reference counterpart: the reference runs one
  * lattice per process (src/evolve.cu:265-270 keeps per-process statics). */
 typedef struct mc_batch mc_batch;
 int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_aos, mc_batch **out);
@@ -211,7 +212,15 @@ int mc_batch_measure_series(mc_batch *b, int n, double *E, double *S);
 int mc_batch_get_directors_aos(mc_batch *b, int rep, float *ni_aos);
 /* Per-replica accepted count of the last sweep, sweep counter, ang_var. */
 int mc_batch_get_state(mc_batch *b, uint32_t *accepted, uint32_t *sweeps, float *ang_var);
+/* Segment-kernel launch shape (grid 0 = default); selects the segment kernel. */
 int mc_batch_set_tuning(mc_batch *b, int grid, int xcd);
+/* Sweep kernel of the batch: MC_KERNEL_AUTO (the slab kernel where every
+ * replica has a tile shape, else the segment kernel), MC_KERNEL_SLAB or
+ * MC_KERNEL_X4.  Both give every replica the same chain. */
+int mc_batch_set_kernel(mc_batch *b, int kind);
+int mc_batch_get_kernel(mc_batch *b, int *kind);
+/* Times segment grids and slab shapes on scratch copies and keeps the
+ * fastest; grid = its workgroup count, xcd its tile schedule. */
 int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us);
 
 /* ------------------------------------------------------------------------

@@ -71,6 +71,18 @@ struct mc_batch {
   float *staging = nullptr;
   std::map<int, hipGraphExec_t> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Slab sweeps: a second plane set (flip tells which allocation holds the
+  // current lattice), the workgroup and tile-shape tables of the current
+  // shape, and the shape's parameters (thread budget, planes per tile).
+  int kind = MC_KERNEL_AUTO;  // requested sweep kernel
+  bool slab = false;          // active: slab (true) or segment (false)
+  float *bx = nullptr, *by = nullptr, *bz = nullptr;
+  int flip = 0;
+  int4 *d_wg = nullptr;
+  int2 *d_geo = nullptr;
+  MCBatchSlab S{};
+  int slab_budget = 0, slab_cz = 0;
+  size_t lds_limit = 65536;
 
   MCBatchLaunch launch() const {
     MCBatchLaunch B;
@@ -100,7 +112,8 @@ void free_batch(mc_batch *b) {
   if (!b) return;
   for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   void *ptrs[] = {b->d_params, b->d_states, b->d_tile_rep, b->d_rep_tile0, b->d_rep_ntiles, b->d_rep_off, b->nx,
-                  b->ny, b->nz, b->d_done, b->d_partials, b->d_out, b->d_series, b->staging};
+                  b->ny, b->nz, b->d_done, b->d_partials, b->d_out, b->d_series, b->staging, b->bx, b->by, b->bz,
+                  b->d_wg, b->d_geo};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -115,26 +128,154 @@ int check_batch(mc_batch *b) {
   return MC_OK;
 }
 
+void swap_planes(mc_batch *b) {
+  std::swap(b->nx, b->bx);
+  std::swap(b->ny, b->by);
+  std::swap(b->nz, b->bz);
+  b->flip ^= 1;
+}
+
+void drop_graphs(mc_batch *b) {
+  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+
+// Tile shape of every replica for a thread budget and a planes-per-tile
+// target: ty the largest divisor of Ny with (Nx/4)(ty+4) <= budget, cz the
+// largest divisor of Nz <= cz_target, both checked as for one lattice.
+// Fills the host tables; false when some replica has no such shape.
+bool slab_plan(const mc_batch *b, int budget, int cz_target, std::vector<int4> &wg, std::vector<int2> &geo,
+               int *threads, size_t *lds) {
+  wg.clear();
+  geo.assign(b->nrep, int2{0, 0});
+  *threads = 0;
+  *lds = 0;
+  for (int r = 0; r < b->nrep; ++r) {
+    const MCDevParams &P = b->hp[r];
+    const int S = P.nx / 4;
+    int ty = 0, cz = 0;
+    for (int t = P.ny; t >= 1; --t)
+      if (P.ny % t == 0 && S * (t + 4) <= budget) {
+        ty = t;
+        break;
+      }
+    for (int c = std::min(cz_target, P.nz); c >= 1; --c)
+      if (P.nz % c == 0) {
+        cz = c;
+        break;
+      }
+    const MCSlabShape sh{ty, cz, 0};
+    if (ty < 1 || cz < 1 || !mc_slab_shape_ok(P, sh, b->lds_limit)) return false;
+    geo[r] = int2{ty, cz};
+    *threads = std::max(*threads, S * (ty + 4));
+    *lds = std::max(*lds, mc_slab_lds_bytes(P, ty));
+    for (int z0 = 0; z0 < P.nz; z0 += cz)
+      for (int y0 = 0; y0 < P.ny; y0 += ty) wg.push_back(int4{r, y0, z0, 0});
+  }
+  *threads = (*threads + 63) / 64 * 64;
+  return true;
+}
+
+// Makes (budget, cz_target) the slab shape: uploads the tables.
+int set_slab_shape(mc_batch *b, int budget, int cz_target) {
+  std::vector<int4> wg;
+  std::vector<int2> geo;
+  int threads = 0;
+  size_t lds = 0;
+  if (!slab_plan(b, budget, cz_target, wg, geo, &threads, &lds))
+    return bfail(MC_EINVAL, "batched slab: no tile shape fits every replica");
+  if (b->d_wg) MB_HIP(hipFree(b->d_wg));
+  b->d_wg = nullptr;
+  MB_HIP(hipMalloc(&b->d_wg, wg.size() * sizeof(int4)));
+  if (!b->d_geo) MB_HIP(hipMalloc(&b->d_geo, b->nrep * sizeof(int2)));
+  MB_HIP(hipMemcpy(b->d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice));
+  MB_HIP(hipMemcpy(b->d_geo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice));
+  b->S = MCBatchSlab{b->d_wg, b->d_geo, (int)wg.size(), threads, lds};
+  b->slab_budget = budget;
+  b->slab_cz = cz_target;
+  (void)mc_slab_prepare(lds);
+  return MC_OK;
+}
+
+// Resolves the requested kernel: the slab when every replica has a tile
+// shape and the second plane set can be allocated, else the segment kernel.
+int select_kernel(mc_batch *b, int kind) {
+  bool want = kind == MC_KERNEL_SLAB || kind == MC_KERNEL_AUTO;
+  if (want && b->slab_budget == 0) {
+    std::vector<int4> wg;
+    std::vector<int2> geo;
+    int threads = 0;
+    size_t lds = 0;
+    if (slab_plan(b, 256, 16, wg, geo, &threads, &lds)) {
+      const int rc = set_slab_shape(b, 256, 16);
+      if (rc) return rc;
+    } else {
+      want = false;
+    }
+  }
+  if (want && !b->bx) {
+    const size_t fb = (size_t)b->total_sites * sizeof(float);
+    float *p[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < 3; ++d)
+      if (hipMalloc(&p[d], fb) != hipSuccess) {
+        (void)hipGetLastError();
+        for (int e = 0; e < 3; ++e)
+          if (p[e]) (void)hipFree(p[e]);
+        want = false;
+        break;
+      }
+    if (want) {
+      b->bx = p[0];
+      b->by = p[1];
+      b->bz = p[2];
+    }
+  }
+  if (kind == MC_KERNEL_SLAB && !want)
+    return bfail(MC_EINVAL, "mc_batch_set_kernel: no slab tile shape fits every replica (or no memory for the "
+                            "second plane set)");
+  b->kind = kind;
+  b->slab = want;
+  drop_graphs(b);
+  return MC_OK;
+}
+
 void enqueue_batch_sweep(mc_batch *b) {
   const MCBatchLaunch B = b->launch();
+  if (b->slab) {
+    mc_launch_batch_slab(B, b->S, b->bx, b->by, b->bz);
+    swap_planes(b);
+    return;
+  }
   mc_launch_batch_x4(B, 0, b->sweep_grid(), 0);
   mc_launch_batch_x4(B, 1, b->sweep_grid(), 1);
 }
 
+// Graph of n sweeps from the current plane set.  Capture records launches
+// without running them, so the plane swaps made while capturing are undone;
+// launch_graph applies them as the replay does.
 int get_graph(mc_batch *b, int n, hipGraphExec_t *out) {
-  auto it = b->graphs.find(n);
+  const int key = 2 * n + b->flip;
+  auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
+    const int flip0 = b->flip;
     hipGraph_t g = nullptr;
     MB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int w = 0; w < n; ++w) enqueue_batch_sweep(b);
+    if (b->flip != flip0) swap_planes(b);
     MB_HIP(hipStreamEndCapture(b->stream, &g));
     hipGraphExec_t ge = nullptr;
     hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     MB_HIP(e);
-    it = b->graphs.emplace(n, ge).first;
+    it = b->graphs.emplace(key, ge).first;
   }
   *out = it->second;
+  return MC_OK;
+}
+
+int launch_graph(mc_batch *b, int n, hipGraphExec_t ge) {
+  MB_HIP(hipGraphLaunch(ge, b->stream));
+  if (b->slab && (n & 1)) swap_planes(b);
   return MC_OK;
 }
 
@@ -186,6 +327,7 @@ int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_ao
   hipDeviceProp_t prop;
   MB_HIPC(hipGetDeviceProperties(&prop, b->device));
   b->cu_count = prop.multiProcessorCount;
+  b->lds_limit = prop.sharedMemPerBlock;
   MB_HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   MB_HIPC(hipEventCreate(&b->ev0));
   MB_HIPC(hipEventCreate(&b->ev1));
@@ -257,6 +399,8 @@ int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_ao
   (void)hipFree(tmp);
   MB_HIPC(e1);
 #undef MB_HIPC
+  const int krc = select_kernel(b, MC_KERNEL_AUTO);
+  if (krc) return bail(krc);
   *out = b;
   return MC_OK;
 }
@@ -287,8 +431,7 @@ int mc_batch_sweeps(mc_batch *b, int n) {
   hipGraphExec_t ge = nullptr;
   rc = get_graph(b, n, &ge);
   if (rc) return rc;
-  MB_HIP(hipGraphLaunch(ge, b->stream));
-  return MC_OK;
+  return launch_graph(b, n, ge);
 }
 
 int mc_batch_sync(mc_batch *b) {
@@ -306,7 +449,8 @@ int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
   rc = get_graph(b, n, &ge);
   if (rc) return rc;
   MB_HIP(hipEventRecord(b->ev0, b->stream));
-  MB_HIP(hipGraphLaunch(ge, b->stream));
+  rc = launch_graph(b, n, ge);
+  if (rc) return rc;
   MB_HIP(hipEventRecord(b->ev1, b->stream));
   MB_HIP(hipEventSynchronize(b->ev1));
   MB_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
@@ -340,10 +484,11 @@ int mc_batch_measure_series(mc_batch *b, int n, double *E, double *S) {
     MB_HIP(hipMalloc(&b->d_series, need * sizeof(double)));
     b->series_cap = (int)need;
   }
-  const MCBatchLaunch B = b->launch();
   for (int w = 0; w < n; ++w) {
     enqueue_batch_sweep(b);
-    mc_launch_batch_measure(B, b->cu_count * 8, b->d_partials, b->d_out, b->d_series + 2 * (size_t)w * b->nrep);
+    // (after the sweep: the slab kernel moves the lattice to the other plane set)
+    mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out,
+                            b->d_series + 2 * (size_t)w * b->nrep);
   }
   MB_HIP(hipGetLastError());
   std::vector<double> o(need);
@@ -390,71 +535,156 @@ int mc_batch_set_tuning(mc_batch *b, int grid, int xcd) {
   if (grid < 0 || (xcd != 0 && xcd != 1) || (xcd && grid % 8))
     return bfail(MC_EINVAL, "mc_batch_set_tuning: grid must be >= 0 and a multiple of 8 with xcd = 1");
   MB_HIP(hipStreamSynchronize(b->stream));
-  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  b->graphs.clear();
+  drop_graphs(b);
   b->grid = grid;
   b->xcd = xcd;
+  b->kind = MC_KERNEL_X4;
+  b->slab = false;
+  return MC_OK;
+}
+
+int mc_batch_set_kernel(mc_batch *b, int kind) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_SLAB && kind != MC_KERNEL_X4)
+    return bfail(MC_EINVAL, "mc_batch_set_kernel: kind must be MC_KERNEL_AUTO, MC_KERNEL_SLAB or MC_KERNEL_X4");
+  MB_HIP(hipStreamSynchronize(b->stream));
+  return select_kernel(b, kind);
+}
+
+int mc_batch_get_kernel(mc_batch *b, int *kind) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!kind) return bfail(MC_EINVAL, "mc_batch_get_kernel: null output");
+  *kind = b->slab ? MC_KERNEL_SLAB : MC_KERNEL_X4;
   return MC_OK;
 }
 
 int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us) {
+  // Candidates sweep scratch copies of the planes and states, so tuning
+  // leaves every chain untouched: segment-kernel grids, then (when the batch
+  // may use it) slab shapes over thread budgets and planes per tile.
   int rc = check_batch(b);
   if (rc) return rc;
   const size_t nt = (size_t)b->total_sites;
+  const bool try_slab = b->kind != MC_KERNEL_X4 && b->bx != nullptr;
   float *scratch = nullptr;
   MCDevState *sst = nullptr;
-  MB_HIP(hipMalloc(&scratch, 3 * nt * sizeof(float)));
-  if (hipMalloc(&sst, b->nrep * sizeof(MCDevState)) != hipSuccess) {
+  int4 *twg = nullptr;
+  int2 *tgeo = nullptr;
+  MB_HIP(hipMalloc(&scratch, (try_slab ? 6 : 3) * nt * sizeof(float)));
+  auto release = [&]() {
     (void)hipFree(scratch);
+    if (sst) (void)hipFree(sst);
+    if (twg) (void)hipFree(twg);
+    if (tgeo) (void)hipFree(tgeo);
+  };
+  if (hipMalloc(&sst, b->nrep * sizeof(MCDevState)) != hipSuccess ||
+      hipMalloc(&tgeo, b->nrep * sizeof(int2)) != hipSuccess) {
+    (void)hipGetLastError();
+    release();
     return bfail(MC_ENOMEM, "mc_batch_autotune: scratch allocation failed");
   }
   MCBatchLaunch B = b->launch();
-  B.nx = scratch;
-  B.ny = scratch + nt;
-  B.nz = scratch + 2 * nt;
   B.states = sst;
-  int best_g = 0, best_x = 0;
+  hipError_t err = hipSuccess;
+  auto reset = [&]() {
+    const float *src[3] = {b->nx, b->ny, b->nz};
+    for (int d = 0; d < 3 && err == hipSuccess; ++d)
+      err = hipMemcpyAsync(scratch + d * nt, src[d], nt * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(sst, b->d_states, b->nrep * sizeof(MCDevState), hipMemcpyDeviceToDevice, b->stream);
+  };
+  auto time_it = [&](auto &&sweep) {
+    for (int w = 0; w < 2; ++w) sweep();
+    const int reps = 6;
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventRecord(b->ev0, b->stream);
+    for (int w = 0; w < reps; ++w) sweep();
+    if (err == hipSuccess) err = hipGetLastError();
+    if (err == hipSuccess) err = hipEventRecord(b->ev1, b->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(b->ev1);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    return ms * 1e3f / reps;
+  };
+  int best_g = 0, best_x = 0, best_budget = 0, best_cz = 0;
   float best = 1e30f;
   const int mults[] = {2, 4, 8, 16, 32};
-  hipError_t err = hipSuccess;
   for (int xm = 0; xm < 2 && err == hipSuccess; ++xm)
     for (int m : mults) {
       const int g = b->cu_count * m;
       if (xm && g % 8) continue;
       B.xcd = xm;
-      err = hipMemcpyAsync(scratch, b->nx, nt * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
-      if (err == hipSuccess) err = hipMemcpyAsync(scratch + nt, b->ny, nt * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
-      if (err == hipSuccess) err = hipMemcpyAsync(scratch + 2 * nt, b->nz, nt * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
-      if (err == hipSuccess)
-        err = hipMemcpyAsync(sst, b->d_states, b->nrep * sizeof(MCDevState), hipMemcpyDeviceToDevice, b->stream);
+      B.nx = scratch;
+      B.ny = scratch + nt;
+      B.nz = scratch + 2 * nt;
+      reset();
       if (err != hipSuccess) break;
-      for (int w = 0; w < 2; ++w) {
+      const float us = time_it([&]() {
         mc_launch_batch_x4(B, 0, g, 0);
         mc_launch_batch_x4(B, 1, g, 1);
-      }
-      const int reps = 6;
-      err = hipEventRecord(b->ev0, b->stream);
-      for (int w = 0; w < reps; ++w) {
-        mc_launch_batch_x4(B, 0, g, 0);
-        mc_launch_batch_x4(B, 1, g, 1);
-      }
-      if (err == hipSuccess) err = hipEventRecord(b->ev1, b->stream);
-      if (err == hipSuccess) err = hipEventSynchronize(b->ev1);
-      float ms = 0.0f;
-      if (err == hipSuccess) err = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      });
       if (err != hipSuccess) break;
-      const float us = ms * 1e3f / reps;
       if (us < best) {
         best = us;
         best_g = g;
         best_x = xm;
       }
     }
-  (void)hipFree(scratch);
-  (void)hipFree(sst);
+  if (try_slab) {
+    const int budgets[] = {128, 192, 256, 384, 512, 768};
+    const int czs[] = {8, 16, 32};
+    for (int budget : budgets)
+      for (int czt : czs) {
+        if (err != hipSuccess) break;
+        std::vector<int4> wg;
+        std::vector<int2> geo;
+        int threads = 0;
+        size_t lds = 0;
+        if (!slab_plan(b, budget, czt, wg, geo, &threads, &lds)) continue;
+        if (twg) (void)hipFree(twg);
+        twg = nullptr;
+        err = hipMalloc(&twg, wg.size() * sizeof(int4));
+        if (err == hipSuccess)
+          err = hipMemcpy(twg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemcpy(tgeo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice);
+        if (err != hipSuccess) break;
+        (void)mc_slab_prepare(lds);
+        const MCBatchSlab S{twg, tgeo, (int)wg.size(), threads, lds};
+        reset();
+        if (err != hipSuccess) break;
+        int cur = 0;
+        B.xcd = 0;
+        const float us = time_it([&]() {
+          float *io = scratch + (size_t)cur * 3 * nt, *oo = scratch + (size_t)(cur ^ 1) * 3 * nt;
+          B.nx = io;
+          B.ny = io + nt;
+          B.nz = io + 2 * nt;
+          mc_launch_batch_slab(B, S, oo, oo + nt, oo + 2 * nt);
+          cur ^= 1;
+        });
+        if (err != hipSuccess) break;
+        if (us < best) {
+          best = us;
+          best_budget = budget;
+          best_cz = czt;
+          best_g = (int)wg.size();
+          best_x = 0;
+        }
+      }
+  }
+  release();
   MB_HIP(err);
-  rc = mc_batch_set_tuning(b, best_g, best_x);
-  if (rc) return rc;
+  if (best_budget > 0) {
+    MB_HIP(hipStreamSynchronize(b->stream));
+    rc = set_slab_shape(b, best_budget, best_cz);
+    if (rc) return rc;
+    b->slab = true;
+    drop_graphs(b);
+  } else {
+    rc = mc_batch_set_tuning(b, best_g, best_x);
+    if (rc) return rc;
+  }
   if (grid) *grid = best_g;
   if (xcd) *xcd = best_x;
   if (best_us) *best_us = best;

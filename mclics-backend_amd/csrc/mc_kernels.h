@@ -72,6 +72,17 @@ struct MCBatchLaunch {
   int xcd;  // tile schedule (mcd_tiles)
 };
 void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finalize);
+// Per-replica ang_var adaptation and sweep advance after a batched sweep.
+void mc_launch_batch_finish(const MCBatchLaunch &B);
+// Batched slab sweep (mc_sweep_slab.hip): one launch per sweep of every
+// replica, reading B.nx.. and writing o*, then the batch finisher.
+struct MCBatchSlab {
+  const int4 *wg;   // per workgroup {replica, y0, z0, 0}
+  const int2 *geo;  // per replica {ty, cz}
+  int nwg, threads;
+  size_t lds;
+};
+void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *ox, float *oy, float *oz);
 // Per-replica FP64 {E, Qxx..Qzz, n} partial sums per tile, then per-replica
 // reduction in tile order: out[r*8..] = {E_mean, S, q6}.
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);

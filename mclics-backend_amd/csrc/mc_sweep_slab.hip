@@ -149,13 +149,7 @@ __device__ __forceinline__ uint32_t slab_block_sum(uint32_t v) {
   return t;
 }
 
-template <int POT, bool RECORD>
-__global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const float *__restrict__ inx,
-                                                            const float *__restrict__ iny,
-                                                            const float *__restrict__ inz, float *__restrict__ outx,
-                                                            float *__restrict__ outy, float *__restrict__ outz,
-                                                            MCDevState *st, int TY, int CZ, int xcd, MCRecord rec) {
-  extern __shared__ float lds[];
+__device__ __forceinline__ SlabGeom slab_geom(const MCDevParams &P, int TY, int CZ) {
   SlabGeom g;
   g.X = P.nx;
   g.Y = P.ny;
@@ -166,15 +160,18 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
   g.R = TY + 4;
   g.X2 = P.nx >> 1;
   g.XY = P.nx * P.ny;
-  const float *in[3] = {inx, iny, inz};
-  float *out[3] = {outx, outy, outz};
-  const mcsu::PassConst kc = mcsu::pass_const(P, st);
+  return g;
+}
 
-  const int nyt = g.Y / TY, nzt = g.Z / CZ, ntiles = nyt * nzt;
-  int tile = blockIdx.x;
-  if (xcd && (ntiles & 7) == 0) tile = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
-  const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
-
+// One tile (rows y0.., planes z0..) of one lattice: the whole sweep of the
+// tile described at the top of this file.  Returns this thread's accepted
+// count (owned sites only); every thread of the workgroup must call it.
+template <int POT, bool RECORD>
+__device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
+                                              const float *const in[3], float *const out[3], int y0, int z0,
+                                              const MCRecord &rec) {
+  extern __shared__ float lds[];
+  const int CZ = g.CZ;
   const int q = threadIdx.x % g.S, r = threadIdx.x / g.S;
   const bool active = r < g.R;  // threads beyond S*R only join barriers
   const int y = ((y0 - 2 + r) % g.Y + g.Y) % g.Y;
@@ -244,8 +241,53 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
     copy3(P2, P3);
     copy3(P3, P4);
   }
+  return acc;
+}
+
+template <int POT, bool RECORD>
+__global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const float *__restrict__ inx,
+                                                            const float *__restrict__ iny,
+                                                            const float *__restrict__ inz, float *__restrict__ outx,
+                                                            float *__restrict__ outy, float *__restrict__ outz,
+                                                            MCDevState *st, int TY, int CZ, int xcd, MCRecord rec) {
+  const SlabGeom g = slab_geom(P, TY, CZ);
+  const float *const in[3] = {inx, iny, inz};
+  float *const out[3] = {outx, outy, outz};
+  const mcsu::PassConst kc = mcsu::pass_const(P, st);
+  const int nyt = g.Y / TY, nzt = g.Z / CZ, ntiles = nyt * nzt;
+  int tile = blockIdx.x;
+  if (xcd && (ntiles & 7) == 0) tile = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
+  const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
+  const uint32_t acc = slab_tile<POT, RECORD>(P, kc, g, in, out, y0, z0, rec);
   const uint32_t bsum = slab_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, true);
+}
+
+// Batched replicas: workgroup b sweeps tile wg[b] = {replica, y0, z0} of its
+// replica with that replica's tile shape geo[r] = {ty, cz}; the replicas'
+// planes sit at rep_off[r] in one plane set (read) and another (written).
+// Each replica evolves exactly as alone.  Counts go to the replica's
+// acc_count; the batch finisher adapts ang_var after the launch.
+template <int POT>
+__global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *__restrict__ params, MCDevState *states,
+                                                            const int4 *__restrict__ wg, const int2 *__restrict__ geo,
+                                                            const long long *__restrict__ rep_off,
+                                                            const float *__restrict__ inx, const float *__restrict__ iny,
+                                                            const float *__restrict__ inz, float *__restrict__ outx,
+                                                            float *__restrict__ outy, float *__restrict__ outz) {
+  const int4 d = wg[blockIdx.x];
+  const int r = d.x;
+  const MCDevParams P = params[r];
+  MCDevState *st = states + r;
+  const int2 sh = geo[r];
+  const SlabGeom g = slab_geom(P, sh.x, sh.y);
+  const long long off = rep_off[r];
+  const float *const in[3] = {inx + off, iny + off, inz + off};
+  float *const out[3] = {outx + off, outy + off, outz + off};
+  const mcsu::PassConst kc = mcsu::pass_const(P, st);
+  const uint32_t acc = slab_tile<POT, false>(P, kc, g, in, out, d.y, d.z, MCRecord{});
+  const uint32_t bsum = slab_block_sum(acc);
+  if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
 }
 
 template <int POT>
@@ -278,6 +320,9 @@ int mc_slab_prepare(size_t lds_bytes) {
   // already the full LDS; that is not an error (a launch beyond the limit
   // still fails cleanly and is reported by the caller).
   const void *fns[] = {
+      reinterpret_cast<const void *>(&k_batch_slab<MCD_LL>),
+      reinterpret_cast<const void *>(&k_batch_slab<MCD_GHRL>),
+      reinterpret_cast<const void *>(&k_batch_slab<MCD_PEAR>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false>),
@@ -307,4 +352,23 @@ void mc_launch_sweep_slab(const MCLaunch &L, float *ox, float *oy, float *oz, co
       launch_t<MCD_LL>(L, ox, oy, oz, s, rec);
       break;
   }
+}
+
+void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *ox, float *oy, float *oz) {
+#define MC_BSLAB(POT_)                                                                                       \
+  k_batch_slab<POT_><<<S.nwg, S.threads, S.lds, B.stream>>>(B.params, B.states, S.wg, S.geo, B.rep_off, B.nx, \
+                                                            B.ny, B.nz, ox, oy, oz)
+  switch (B.potential) {
+    case MCD_GHRL:
+      MC_BSLAB(MCD_GHRL);
+      break;
+    case MCD_PEAR:
+      MC_BSLAB(MCD_PEAR);
+      break;
+    default:
+      MC_BSLAB(MCD_LL);
+      break;
+  }
+#undef MC_BSLAB
+  mc_launch_batch_finish(B);
 }

@@ -330,5 +330,9 @@ void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finali
                                                         B.ntiles, B.nrep, B.nx, B.ny, B.nz, colour, B.xcd);
       break;
   }
-  if (finalize) k_batch_finish<<<1, kBlock, 0, B.stream>>>(B.params, B.states, B.nrep);
+  if (finalize) mc_launch_batch_finish(B);
+}
+
+void mc_launch_batch_finish(const MCBatchLaunch &B) {
+  k_batch_finish<<<1, kBlock, 0, B.stream>>>(B.params, B.states, B.nrep);
 }

@@ -74,6 +74,8 @@ EXPORTS = (
     "mc_batch_get_directors_aos",
     "mc_batch_get_state",
     "mc_batch_set_tuning",
+    "mc_batch_set_kernel",
+    "mc_batch_get_kernel",
     "mc_batch_autotune",
     "mc_mlp_create",
     "mc_mlp_destroy",
@@ -189,6 +191,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_batch_get_directors_aos.argtypes = [vp, C.c_int, f32p]
     lib.mc_batch_get_state.argtypes = [vp, u32p, u32p, f32p]
     lib.mc_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    lib.mc_batch_set_kernel.argtypes = [vp, C.c_int]
+    lib.mc_batch_get_kernel.argtypes = [vp, ip]
     lib.mc_batch_autotune.argtypes = [vp, ip, ip, f32p]
     lib.mc_mlp_create.argtypes = [f32p, C.c_uint64, C.POINTER(vp)]
     lib.mc_mlp_destroy.argtypes = [vp]
@@ -528,6 +532,14 @@ class Batch:
 
     def set_tuning(self, grid: int, xcd: int):
         _check(self._lib.mc_batch_set_tuning(self._h, grid, xcd))
+
+    def set_kernel(self, kind: int) -> None:
+        _check(self._lib.mc_batch_set_kernel(self._h, kind))
+
+    def kernel(self) -> int:
+        v = C.c_int()
+        _check(self._lib.mc_batch_get_kernel(self._h, C.byref(v)))
+        return v.value
 
     def autotune(self):
         g, x, us = C.c_int(), C.c_int(), C.c_float()
