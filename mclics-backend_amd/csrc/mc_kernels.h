@@ -86,6 +86,8 @@ void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *o
 // Per-replica FP64 {E, Qxx..Qzz, n} partial sums per tile, then per-replica
 // reduction in tile order: out[r*8..] = {E_mean, S, q6}.
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);
+// The per-tile partial sums alone (mc_sweep_x4.hip).
+void mc_launch_batch_measure_tiles(const MCBatchLaunch &B, int grid, double *tile_partials);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
 // Local order parameter per site (NaN where pt == 0).
 void mc_launch_local_S(const MCLaunch &L, int grid, float *out);

@@ -359,62 +359,6 @@ __global__ void k_soa_to_aos(const float *__restrict__ x, const float *__restric
   }
 }
 
-// Batched replicas: per-tile FP64 partial sums {E, Qxx..Qzz, n} (a tile is
-// 256 segments = 1024 consecutive sites of one replica).
-template <int POT>
-__global__ __launch_bounds__(kBlock) void k_batch_measure(const MCDevParams *__restrict__ params,
-                                                          const MCDevState *__restrict__ states,
-                                                          const int *__restrict__ tile_rep,
-                                                          const int *__restrict__ rep_tile0,
-                                                          const long long *__restrict__ rep_off, int ntiles,
-                                                          const float *__restrict__ nx, const float *__restrict__ ny,
-                                                          const float *__restrict__ nz, double *__restrict__ partials) {
-  __shared__ double red[8][kBlock / 64];
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int r = tile_rep[tile];
-    const MCDevParams P = params[r];
-    const float elecE = states[r].elecE;
-    const double cA = (double)(-P.A) * 1.5;
-    const long long off = rep_off[r];
-    const float *X = nx + off, *Y = ny + off, *Z = nz + off;
-    const int t = (tile - rep_tile0[r]) * kBlock + threadIdx.x;
-    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (t < (P.nx >> 2) * P.ny * P.nz) {
-      for (int e = 0; e < 4; ++e) {
-        const int s = 4 * t + e;
-        const int i = s % P.nx, j = (s / P.nx) % P.ny, k = s / (P.nx * P.ny);
-        const float o0 = X[s], o1 = Y[s], o2 = Z[s];
-        float Eo, En;
-        mcd_site_energy<POT, false, false>(P, cA, X, Y, Z, nullptr, nullptr, nullptr, nullptr, elecE, i, j, k, s,
-                                           o0, o1, o2, 0.f, 0.f, 0.f, Eo, En);
-        const double x = o0, y = o1, z = o2;
-        a[0] += (double)Eo;
-        a[1] += 3.0 * x * x - 1.0;
-        a[2] += 3.0 * x * y;
-        a[3] += 3.0 * x * z;
-        a[4] += 3.0 * y * y - 1.0;
-        a[5] += 3.0 * y * z;
-        a[6] += 3.0 * z * z - 1.0;
-        a[7] += 1.0;
-      }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      double v = a[c];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-      if (lane == 0) red[c][wid] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-      double v = 0;
-      for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
-      partials[(size_t)tile * 8 + threadIdx.x] = v;
-    }
-    __syncthreads();
-  }
-}
-
 // One workgroup per replica sums its tiles in tile order (deterministic):
 // out[r*8..] = {E_mean, S, q6}; series[r*2..] = {E, S} when given.
 __global__ __launch_bounds__(kBlock) void k_batch_measure_final(const int *__restrict__ rep_tile0,
@@ -538,19 +482,6 @@ void mc_launch_soa_to_aos(hipStream_t s, const float *x, const float *y, const f
 }
 
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series) {
-  switch (B.potential) {
-    case MCD_GHRL:
-      k_batch_measure<MCD_GHRL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                               B.ntiles, B.nx, B.ny, B.nz, tile_partials);
-      break;
-    case MCD_PEAR:
-      k_batch_measure<MCD_PEAR><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                               B.ntiles, B.nx, B.ny, B.nz, tile_partials);
-      break;
-    default:
-      k_batch_measure<MCD_LL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0, B.rep_off,
-                                                             B.ntiles, B.nx, B.ny, B.nz, tile_partials);
-      break;
-  }
+  mc_launch_batch_measure_tiles(B, grid, tile_partials);
   k_batch_measure_final<<<B.nrep, kBlock, 0, B.stream>>>(B.rep_tile0, B.rep_ntiles, tile_partials, out, series);
 }

@@ -117,71 +117,64 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
 // neighbour rows come from L2.  Writes FP64 partials per workgroup
 // {E, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n}; the one-block finisher of the general
 // measurement reduces them in workgroup order (deterministic).
+// Adds the four sites of segment q of row (j, k) to a[8] = {E, 3xx-1, 3xy,
+// 3xz, 3yy-1, 3yz, 3zz-1, n}: each site energy with the reference arithmetic
+// (FP32 bonds summed in FP64, then rounded to FP32), Q terms in FP64.
 template <int POT>
-__global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const float *__restrict__ nx,
-                                                       const float *__restrict__ ny, const float *__restrict__ nz,
-                                                       const MCDevState *st, int xcd,
-                                                       double *__restrict__ partials) {
-  const float elecE = st->elecE;
-  const double cA = (double)(-P.A) * 1.5;
+__device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA, float elecE,
+                                                const float *const pl[3], int q, int j, int k, double a[8]) {
   const int X = P.nx, Y = P.ny, Z = P.nz;
   const int nseg = X >> 2;
-  const int total = nseg * Y * Z;
-  const float *pl[3] = {nx, ny, nz};
-  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const MCTileRange tr = mcd_tiles((total + kBlock - 1) / kBlock, xcd);
-  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-    const int t = tile * kBlock + threadIdx.x;
-    if (t >= total) continue;
-    const int q = t % nseg, rowi = t / nseg;
-    const int j = rowi % Y, k = rowi / Y;
-    const int jp = j + 1 == Y ? 0 : j + 1, jm = j == 0 ? Y - 1 : j - 1;
-    const int kp = k + 1 == Z ? 0 : k + 1, km = k == 0 ? Z - 1 : k - 1;
-    const int row = X * (j + Y * k), x0 = 4 * q;
-    const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
-    const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
-    const int xl = row + (q == 0 ? X - 1 : x0 - 1), xr = row + (q == nseg - 1 ? 0 : x0 + 4);
-    float4 c[3], yp[3], ym[3], zp[3], zm[3];
-    float ol[3], orr[3];
+  const int jp = j + 1 == Y ? 0 : j + 1, jm = j == 0 ? Y - 1 : j - 1;
+  const int kp = k + 1 == Z ? 0 : k + 1, km = k == 0 ? Z - 1 : k - 1;
+  const int row = X * (j + Y * k), x0 = 4 * q;
+  const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
+  const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
+  const int xl = row + (q == 0 ? X - 1 : x0 - 1), xr = row + (q == nseg - 1 ? 0 : x0 + 4);
+  float4 c[3], yp[3], ym[3], zp[3], zm[3];
+  float ol[3], orr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    c[d] = *reinterpret_cast<const float4 *>(pl[d] + row + x0);
+    yp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
+    ym[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
+    zp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
+    zm[d] = *reinterpret_cast<const float4 *>(pl[d] + r_km);
+    ol[d] = pl[d][xl];
+    orr[d] = pl[d][xr];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    float o[3], m[6][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      c[d] = *reinterpret_cast<const float4 *>(pl[d] + row + x0);
-      yp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
-      ym[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
-      zp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
-      zm[d] = *reinterpret_cast<const float4 *>(pl[d] + r_km);
-      ol[d] = pl[d][xl];
-      orr[d] = pl[d][xr];
+      o[d] = sel(c[d], e);
+      m[0][d] = e == 3 ? orr[d] : sel(c[d], e + 1);  // +x
+      m[1][d] = e == 0 ? ol[d] : sel(c[d], e - 1);   // -x
+      m[2][d] = sel(yp[d], e);
+      m[3][d] = sel(ym[d], e);
+      m[4][d] = sel(zp[d], e);
+      m[5][d] = sel(zm[d], e);
     }
+    double so = 0.0;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float o[3], m[6][3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        o[d] = sel(c[d], e);
-        m[0][d] = e == 3 ? orr[d] : sel(c[d], e + 1);  // +x
-        m[1][d] = e == 0 ? ol[d] : sel(c[d], e - 1);   // -x
-        m[2][d] = sel(yp[d], e);
-        m[3][d] = sel(ym[d], e);
-        m[4][d] = sel(zp[d], e);
-        m[5][d] = sel(zm[d], e);
-      }
-      double so = 0.0;
-#pragma unroll
-      for (int b = 0; b < 6; ++b) so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
-      float E = (float)so;
-      if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
-      const double x = o[0], y = o[1], z = o[2];
-      a[0] += (double)E;
-      a[1] += 3.0 * x * x - 1.0;
-      a[2] += 3.0 * x * y;
-      a[3] += 3.0 * x * z;
-      a[4] += 3.0 * y * y - 1.0;
-      a[5] += 3.0 * y * z;
-      a[6] += 3.0 * z * z - 1.0;
-      a[7] += 1.0;
-    }
+    for (int b = 0; b < 6; ++b) so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+    float E = (float)so;
+    if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
+    const double x = o[0], y = o[1], z = o[2];
+    a[0] += (double)E;
+    a[1] += 3.0 * x * x - 1.0;
+    a[2] += 3.0 * x * y;
+    a[3] += 3.0 * x * z;
+    a[4] += 3.0 * y * y - 1.0;
+    a[5] += 3.0 * y * z;
+    a[6] += 3.0 * z * z - 1.0;
+    a[7] += 1.0;
   }
+}
+
+// Sum over the workgroup of a[8]; threads 0..7 get component threadIdx.x.
+__device__ __forceinline__ double block_sum8(const double a[8]) {
   __shared__ double red[8][kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -191,10 +184,64 @@ __global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const floa
     if (lane == 0) red[c][wid] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 8) {
-    double v = 0;
+  double v = 0;
+  if (threadIdx.x < 8)
     for (int w = 0; w < kBlock / 64; ++w) v += red[threadIdx.x][w];
-    partials[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+  __syncthreads();  // red is reused by the next call
+  return v;
+}
+
+template <int POT>
+__global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const float *__restrict__ nx,
+                                                       const float *__restrict__ ny, const float *__restrict__ nz,
+                                                       const MCDevState *st, int xcd,
+                                                       double *__restrict__ partials) {
+  const float elecE = st->elecE;
+  const double cA = (double)(-P.A) * 1.5;
+  const int nseg = P.nx >> 2;
+  const int total = nseg * P.ny * P.nz;
+  const float *const pl[3] = {nx, ny, nz};
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const MCTileRange tr = mcd_tiles((total + kBlock - 1) / kBlock, xcd);
+  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int t = tile * kBlock + threadIdx.x;
+    if (t >= total) continue;
+    const int q = t % nseg, rowi = t / nseg;
+    measure_segment<POT>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
+  }
+  const double v = block_sum8(a);
+  if (threadIdx.x < 8) partials[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+}
+
+// Batched replicas: FP64 partial sums {E, Qxx..Qzz, n} per tile (256
+// segments of one replica), reduced per replica in tile order by
+// k_batch_measure_final (mc_kernels.hip).
+template <int POT>
+__global__ __launch_bounds__(kBlock) void k_batch_measure_x4(const MCDevParams *__restrict__ params,
+                                                             const MCDevState *__restrict__ states,
+                                                             const int *__restrict__ tile_rep,
+                                                             const int *__restrict__ rep_tile0,
+                                                             const long long *__restrict__ rep_off, int ntiles,
+                                                             const float *__restrict__ nx,
+                                                             const float *__restrict__ ny,
+                                                             const float *__restrict__ nz,
+                                                             double *__restrict__ partials) {
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int r = tile_rep[tile];
+    const MCDevParams P = params[r];
+    const float elecE = states[r].elecE;
+    const double cA = (double)(-P.A) * 1.5;
+    const long long off = rep_off[r];
+    const float *const pl[3] = {nx + off, ny + off, nz + off};
+    const int nseg = P.nx >> 2;
+    const int t = (tile - rep_tile0[r]) * kBlock + threadIdx.x;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < nseg * P.ny * P.nz) {
+      const int q = t % nseg, rowi = t / nseg;
+      measure_segment<POT>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
+    }
+    const double v = block_sum8(a);
+    if (threadIdx.x < 8) partials[(size_t)tile * 8 + threadIdx.x] = v;
   }
 }
 
@@ -214,6 +261,23 @@ bool mc_x4_eligible(const MCLaunch &L) {
 }
 
 int mc_x4_slots(const MCDevParams &P) { return (P.nx / 4) * P.ny * P.nz; }
+
+void mc_launch_batch_measure_tiles(const MCBatchLaunch &B, int grid, double *tile_partials) {
+  switch (B.potential) {
+    case MCD_GHRL:
+      k_batch_measure_x4<MCD_GHRL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0,
+                                                                  B.rep_off, B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+    case MCD_PEAR:
+      k_batch_measure_x4<MCD_PEAR><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0,
+                                                                  B.rep_off, B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+    default:
+      k_batch_measure_x4<MCD_LL><<<grid, kBlock, 0, B.stream>>>(B.params, B.states, B.tile_rep, B.rep_tile0,
+                                                                B.rep_off, B.ntiles, B.nx, B.ny, B.nz, tile_partials);
+      break;
+  }
+}
 
 void mc_launch_measure_x4(const MCLaunch &L, int grid, int xcd, double *partials, double *out, double *series) {
   switch (L.P.potential) {
