@@ -100,6 +100,7 @@ def learned_leg(n_meas=30000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14
                         ms, _ = lat.time_sweeps(300, graph=False)
                         sps = 300 / (ms / 1e3)
                     else:
+                        lat.autotune()  # the plain chain at its fastest kernel shape for this size
                         lat.sweeps(200, graph=True)
                         ms, _ = lat.time_sweeps(1000, graph=True)
                         sps = 1000 / (ms / 1e3)
@@ -118,6 +119,31 @@ def learned_leg(n_meas=30000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14
             out["per_T"].append(row)
         out["gain_geomean"] = float(np.exp(np.mean(np.log(gains))))
     return out
+
+
+def batch_leg(n_sweeps=400):
+    """Config C4: 64 independent periodic LL replicas (16 each of 32^3, 48^3,
+    64^3 and 96^3; 20,643,840 sites), T_r uniform in [0.9, 1.3], IC seeds
+    100..163, swept together (one batched launch per sweep, replayed as a
+    graph)."""
+    import mclics
+    sizes = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
+    rng = np.random.default_rng(0)
+    cfgs = [mclics.Config(s, s, s, T=float(np.float32(rng.uniform(0.9, 1.3))), seed=100 + r, stream=r)
+            for r, s in enumerate(sizes)]
+    ics = [mclics.random_ic(s, s, s, seed=100 + r) for r, s in enumerate(sizes)]
+    tot = sum(s**3 for s in sizes)
+    with mclics.Batch(cfgs, ics) as b:
+        b.sweeps(20)
+        b.sync()
+        ms = b.time_sweeps(n_sweeps)
+        kernel = {mclics.KERNEL_SLAB: "k_batch_slab", mclics.KERNEL_X4: "k_batch_x4"}.get(b.kernel(), "?")
+        E, S = b.measure()
+    rate = n_sweeps * tot / (ms / 1e3)
+    return {"workload": "64 LL replicas periodic: 16 each of 32^3, 48^3, 64^3, 96^3; T_r ~ U[0.9, 1.3]",
+            "sites": tot, "sweeps": n_sweeps, "us_per_sweep": ms * 1e3 / n_sweeps, "kernel": kernel,
+            "site_updates_per_s": rate, "roofline_frac_24B": rate * BYTES_PER_SITE_UPDATE / (HBM_PEAK_GBS * 1e9),
+            "S_mean": float(np.mean(S)), "E_mean": float(np.mean(E))}
 
 
 def load_traffic(kernel):
@@ -157,6 +183,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
     ap.add_argument("--learned-sweeps", type=int, default=30000,
                     help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
@@ -241,6 +268,9 @@ def main():
         "observables": obs,
         "last_acceptance": acc / nt,
     }
+    if not args.no_batch:
+        log("bench: batched-replica leg (64 replicas, 20.6M sites)")
+        result["batched_replicas"] = batch_leg()
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
         result["learned_proposal"] = learned_leg(args.learned_sweeps)
