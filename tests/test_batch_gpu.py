@@ -5,6 +5,7 @@ The single-lattice path is itself pinned to the CPU oracle elsewhere."""
 import numpy as np
 import pytest
 
+import _oracle as O
 import mclics
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +88,36 @@ def test_batch_slab_and_segment_kernels_agree():
         a2, s2, v2 = b2.state()
         assert np.array_equal(a1, a2) and np.array_equal(s1, s2) and np.array_equal(v1, v2)
         assert np.all(s1 == 15)
+
+
+@pytest.mark.parametrize("pot", [O.GHRL, O.PEAR])
+def test_batch_other_potentials_match_single_lattices(pot):
+    """GHRL (chiral, with a field) and pear replicas in a batch, on the slab
+    kernel and on the segment kernel, against the same lattices alone."""
+    sizes = [(8, 8, 8), (12, 6, 10), (16, 8, 12)]
+    kw = {"k": (1.0, 0.7, 1.3, 16.0)} if pot == O.GHRL else {"A": 1.1, "B1": 0.05, "Cc": 0.3}
+    po = O.params(8, 8, 8, potential=pot, elec=(0.2, -0.1, 1.0), elecA=0.7, elecE=0.6, **kw)
+    cfgs, ics = [], []
+    for r, (x, y, z) in enumerate(sizes):
+        cfgs.append(mclics.Config(x, y, z, potential=pot, A=po.A, B1=po.B1, C=po.C, ghrl=O.ghrl_tuple(po),
+                                  elec=(0.2, -0.1, 1.0), elecA=0.7, elecE=0.6, T=0.8 + 0.1 * r, seed=40 + r,
+                                  stream=r))
+        ics.append(mclics.random_ic(x, y, z, seed=40 + r))
+    singles = []
+    for r, cfg in enumerate(cfgs):
+        with mclics.Lattice(cfg, ics[r]) as lat:
+            lat.sweeps(9)
+            singles.append((lat.directors(), lat.acceptance(), lat.ang_var()))
+    for kind in (mclics.KERNEL_SLAB, mclics.KERNEL_X4):
+        with mclics.Batch(cfgs, ics) as b:
+            b.set_kernel(kind)
+            b.sweeps(4)
+            b.sweeps(5)
+            acc, sw, av = b.state()
+            for r in range(len(sizes)):
+                d, (a1, s1), av1 = singles[r]
+                assert np.array_equal(b.directors(r), d), (kind, r)
+                assert (acc[r], sw[r], av[r]) == (a1, s1, av1), (kind, r)
 
 
 def test_batch_rejects_unsupported_replicas():
