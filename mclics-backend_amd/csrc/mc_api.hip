@@ -431,6 +431,30 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   }
   if (valid == 0) return fail(MC_EINVAL, "mc_create: no active site");
   if (any_surface && !ns_aos) return fail(MC_EINVAL, "mc_create: surface sites need surface normals");
+  // Slab geometry for the fast kernels: x, y periodic, z free, every site
+  // active, and either no surface sites or exactly the faces of
+  // src/geometry_slab.cpp:47-67 (pt 2 with normal (0,0,-1) on k = 0, pt 3
+  // with (0,0,+1) on k = Nz-1).
+  int zfree = 0, zsurf = 0;
+  if (p->bound[0] == MC_PERIODIC && p->bound[1] == MC_PERIODIC && p->bound[2] != MC_PERIODIC && p->nz >= 2 &&
+      p->nx % 4 == 0) {
+    const int plane = p->nx * p->ny;
+    bool faces = true;
+    for (int s = 0; s < nt && (all_bulk || faces); ++s) {
+      const int k = s / plane;
+      const int want = k == 0 ? 2 : (k == p->nz - 1 ? 3 : 1);
+      if (pt[s] != want) {
+        faces = false;
+        continue;
+      }
+      if (want > 1) {
+        const float *n = ns_aos + 3 * (size_t)s;
+        if (n[0] != 0.0f || n[1] != 0.0f || n[2] != (want == 2 ? -1.0f : 1.0f)) faces = false;
+      }
+    }
+    if (all_bulk) zfree = 1;
+    else if (faces) zfree = zsurf = 1;
+  }
   if (mc_device_count() < 1) return fail(MC_ENODEV, "mc_create: no gfx950 device visible");
 
   mc_handle *h = new mc_handle();
@@ -469,6 +493,8 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   MC_HIPC(hipEventCreate(&h->ev1));
 
   mc_to_dev_params(p, &h->P);
+  h->P.zfree = zfree;
+  h->P.zsurf = zsurf;
   h->P.nt = nt;
   h->P.valid = valid;
   h->order = p->order;

@@ -71,6 +71,11 @@ struct MCDevParams {
   MCDiv div_hx, div_ny, div_nj, div_ninj;
   // MCD_NSUB sweep sub-counters, MCD_SUB_STRIDE words apart (mcd_finish_sweep)
   unsigned long long *ctr;
+  // Slab geometry on the fast kernels (checked against pt and ns at create):
+  // zfree: z is free (x, y periodic, every site active); zsurf: the planes
+  // k = 0 and k = Nz-1 carry surfaces 0 and 1 with normals (0, 0, -1) and
+  // (0, 0, +1), the layout of src/geometry_slab.cpp:47-67.
+  int zfree, zsurf;
 };
 
 // Sub-counters of the sweep bookkeeping: same-address device atomics

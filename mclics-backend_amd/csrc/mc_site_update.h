@@ -53,17 +53,35 @@ struct Site {
   float u;        // accept uniform
 };
 
+// Where the pair sits: bit b of `on` set when neighbour b exists (a free z
+// face drops one), surf the surface of its plane (-1 for none).  Both sites
+// of a pair share one row, so one Where serves both.
+struct Where {
+  uint32_t on;
+  int surf;
+};
+constexpr Where kBulk{0x3Fu, -1};
+
+// Site energies in the order of the reference's site energy
+// (src/geometry.cpp:19-72): bonds summed in FP64 and rounded, then the
+// surface term, then the field.
 template <int POT>
 __device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, float elecE, const Site &S,
-                                               float &Eo, float &En) {
+                                               const Where &w, float &Eo, float &En) {
   double so = 0.0, sn = 0.0;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
+    if (!((w.on >> b) & 1u)) continue;
     so += (double)mcd_bond<POT>(P, cA, b, S.o[0], S.o[1], S.o[2], S.m[b][0], S.m[b][1], S.m[b][2]);
     sn += (double)mcd_bond<POT>(P, cA, b, S.q[0], S.q[1], S.q[2], S.m[b][0], S.m[b][1], S.m[b][2]);
   }
   Eo = (float)so;
   En = (float)sn;
+  if (w.surf >= 0) {
+    const float sz = w.surf ? 1.0f : -1.0f;
+    Eo += mcd_surface(P, w.surf, S.o[0], S.o[1], S.o[2], 0.0f, 0.0f, sz);
+    En += mcd_surface(P, w.surf, S.q[0], S.q[1], S.q[2], 0.0f, 0.0f, sz);
+  }
   if (P.elecA != 0.0f) {
     Eo += mcd_field(P, elecE, S.o[0], S.o[1], S.o[2]);
     En += mcd_field(P, elecE, S.q[0], S.q[1], S.q[2]);
@@ -128,7 +146,10 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
 // and their difference is <= 114u * 1.5|A|; with the final product, the
 // distance to the reference's dE stays below 300u * 1.5|A|, under a third of
 // eps = 2^-14 * 1.5|A| = 1024u * 1.5|A|.
-__device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S) {
+// An absent neighbour is a zero vector here, whose bond term is exactly 0;
+// a surface site is always decided exactly.
+__device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
+  if (w.surf >= 0) return -1;
   const float d0 = S.q[0] - S.o[0], d1 = S.q[1] - S.o[1], d2 = S.q[2] - S.o[2];
   const float s0 = S.q[0] + S.o[0], s1 = S.q[1] + S.o[1], s2 = S.q[2] + S.o[2];
   float acc = 0.0f;
@@ -154,9 +175,12 @@ struct PairNbrs {
 
 // Proposes and decides both sites of the pair whose first site has linear
 // index sA; c is the segment.  Fills S (for recording) and a.
+// An absent z neighbour (free face) must arrive as a zero vector with its
+// bit clear in w.on.
 template <int POT, int PAR>
 __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassConst &k, const float4 c[3],
-                                            const PairNbrs &nb, uint32_t sA, Site S[2], bool a[2], MCU4 &r) {
+                                            const PairNbrs &nb, uint32_t sA, const Where &w, Site S[2], bool a[2],
+                                            MCU4 &r) {
   r = mcd_philox(MCU4{sA, k.sweep, P.stream, 0x4D44u}, P.seed_lo, P.seed_hi);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -181,7 +205,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   if (POT == MCD_LL) {
     // Screen both sites, then one shared (rarely executed) exact evaluation
     // for whichever sites fell inside the error band.
-    const int d0 = screen_ll(P, k, S[0]), d1 = screen_ll(P, k, S[1]);
+    const int d0 = screen_ll(P, k, S[0], w), d1 = screen_ll(P, k, S[1], w);
     a[0] = d0 == 1;
     a[1] = d1 == 1;
     if (d0 < 0 || d1 < 0) {
@@ -190,7 +214,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
         if ((t ? d1 : d0) >= 0) continue;
         const Site &X = t ? S[1] : S[0];
         float Eo, En;
-        exact_energies<POT>(P, k.cA, k.elecE, X, Eo, En);
+        exact_energies<POT>(P, k.cA, k.elecE, X, w, Eo, En);
         const bool at = mcd_accept(En, Eo, k.T, X.u);
         if (t) a[1] = at; else a[0] = at;
       }
@@ -200,7 +224,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
 #pragma unroll
   for (int t = 0; t < 2; ++t) {  // GHRL and pear: the exact arithmetic throughout
     float Eo, En;
-    exact_energies<POT>(P, k.cA, k.elecE, S[t], Eo, En);
+    exact_energies<POT>(P, k.cA, k.elecE, S[t], w, Eo, En);
     a[t] = mcd_accept(En, Eo, k.T, S[t].u);
   }
 }
@@ -222,14 +246,14 @@ __device__ __forceinline__ void pair_apply(float4 c[3], const Site S[2], const b
 // Record of both sites of a pair (the parity replay reads it back).
 template <int POT>
 __device__ __forceinline__ void pair_record(const MCDevParams &P, const PassConst &k, const MCRecord &rec, int sA,
-                                            const Site S[2], const bool a[2], const MCU4 &r) {
+                                            const Where &w, const Site S[2], const bool a[2], const MCU4 &r) {
   const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
   const uint32_t ax1 = ((r.z & 0xFFu) << 8) | (r.w & 0xFFu);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int s = sA + 2 * t;
     float Eo, En;
-    exact_energies<POT>(P, k.cA, k.elecE, S[t], Eo, En);
+    exact_energies<POT>(P, k.cA, k.elecE, S[t], w, Eo, En);
     rec.u0[s] = mcd_u01(t ? r.z : r.x);
     rec.u1[s] = (float)(t ? ax1 : ax0) * (1.0f / 65536.0f);
     rec.u2[s] = S[t].u;

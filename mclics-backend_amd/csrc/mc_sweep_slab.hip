@@ -53,6 +53,7 @@ struct SlabGeom {
   int R;         // rows held (TY + 4)
   int X2;        // half-row length (X / 2)
   int TY, CZ;
+  int zfree, zsurf;  // slab geometry (MCDevParams)
 };
 
 // Offsets of one thread that stay fixed while the tile walks z.
@@ -66,11 +67,31 @@ struct LaneOff {
 // Plane index z in [-2, Z+2) wrapped onto the periodic lattice.
 __device__ __forceinline__ int wrap_z(const SlabGeom &g, int z) { return z < 0 ? z + g.Z : (z >= g.Z ? z - g.Z : z); }
 
+// Plane z exists: always for periodic z, inside [0, Z) for a free z.
+__device__ __forceinline__ bool has_plane(const SlabGeom &g, int z) { return !g.zfree || (z >= 0 && z < g.Z); }
+
+// A plane beyond a free face reads as zero vectors.
 __device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], const SlabGeom &g, const LaneOff &L,
                                          int z, float4 v[3]) {
+  if (!has_plane(g, z)) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
   const int off = g.XY * wrap_z(g, z) + L.rowbase;  // < Nt < 2^31
 #pragma unroll
   for (int d = 0; d < 3; ++d) v[d] = *reinterpret_cast<const float4 *>(in[d] + off);
+}
+
+// Neighbours and surface of plane z (uniform over the workgroup).
+__device__ __forceinline__ mcsu::Where where_z(const SlabGeom &g, int z) {
+  mcsu::Where w = mcsu::kBulk;
+  if (g.zfree) {
+    if (z + 1 >= g.Z) w.on &= ~(1u << 4);
+    if (z <= 0) w.on &= ~(1u << 5);
+    if (g.zsurf) w.surf = z == 0 ? 0 : (z == g.Z - 1 ? 1 : -1);
+  }
+  return w;
 }
 
 // Stores the two sites of parity par (elements par, par+2) of the segment
@@ -111,12 +132,13 @@ __device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu
     nb.ov[d] = sl[mo];
   }
   const int sA = g.XY * wrap_z(g, z) + L.rowbase + PAR;
+  const mcsu::Where w = where_z(g, z);
   mcsu::Site S[2];
   bool a[2];
   MCU4 rr;
-  mcsu::pair_update<POT, PAR>(P, kc, own, nb, (uint32_t)sA, S, a, rr);
+  mcsu::pair_update<POT, PAR>(P, kc, own, nb, (uint32_t)sA, w, S, a, rr);
   mcsu::pair_apply<PAR>(own, S, a);
-  if (RECORD && counted) mcsu::pair_record<POT>(P, kc, rec, sA, S, a, rr);
+  if (RECORD && counted) mcsu::pair_record<POT>(P, kc, rec, sA, w, S, a, rr);
   return counted ? (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u) : 0u;
 }
 
@@ -160,6 +182,8 @@ __device__ __forceinline__ SlabGeom slab_geom(const MCDevParams &P, int TY, int 
   g.R = TY + 4;
   g.X2 = P.nx >> 1;
   g.XY = P.nx * P.ny;
+  g.zfree = P.zfree;
+  g.zsurf = P.zsurf;
   return g;
 }
 
@@ -206,7 +230,7 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   }
   __syncthreads();
   if (active && red_row) {
-    update<POT, false>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
+    if (has_plane(g, z0 - 1)) update<POT, false>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
     acc += update<POT, RECORD>(P, kc, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
     put_half(Rbuf(z0 & 1), L, ((y + z0) & 1), P1);
   }
@@ -219,7 +243,7 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
     if (active) {
       if (it <= CZ - 2) load_seg(in, g, L, p + 3, P4);
       if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
-      if (red_row) {
+      if (red_row && has_plane(g, p + 1)) {
         acc += update<POT, RECORD>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
                                    owned_row && it <= CZ - 2, rec);
         if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);

@@ -55,6 +55,13 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
   const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
   const int own = row + x0;
   const int out = PAR == 0 ? row + (q == 0 ? X - 1 : x0 - 1) : row + (q == nseg - 1 ? 0 : x0 + 4);
+  // free z faces: the missing neighbour is a zero vector with its bit clear
+  mcsu::Where w = mcsu::kBulk;
+  const bool zp_on = !P.zfree || k + 1 < Z, zm_on = !P.zfree || k > 0;
+  if (!zp_on) w.on &= ~(1u << 4);
+  if (!zm_on) w.on &= ~(1u << 5);
+  if (P.zsurf) w.surf = k == 0 ? 0 : (k == Z - 1 ? 1 : -1);
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float *pl[3] = {nx, ny, nz};
   float4 c[3];
   mcsu::PairNbrs nb;
@@ -63,8 +70,8 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
     c[d] = *reinterpret_cast<const float4 *>(pl[d] + own);
     const float4 yp = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
     const float4 ym = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
-    const float4 zp = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
-    const float4 zm = *reinterpret_cast<const float4 *>(pl[d] + r_km);
+    const float4 zp = zp_on ? *reinterpret_cast<const float4 *>(pl[d] + r_kp) : zero4;
+    const float4 zm = zm_on ? *reinterpret_cast<const float4 *>(pl[d] + r_km) : zero4;
     nb.yp[d] = make_float2(sel(yp, PAR), sel(yp, PAR + 2));
     nb.ym[d] = make_float2(sel(ym, PAR), sel(ym, PAR + 2));
     nb.zp[d] = make_float2(sel(zp, PAR), sel(zp, PAR + 2));
@@ -75,13 +82,13 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
   mcsu::Site S[2];
   bool a[2];
   MCU4 r;
-  mcsu::pair_update<POT, PAR>(P, kc, c, nb, (uint32_t)sA, S, a, r);
+  mcsu::pair_update<POT, PAR>(P, kc, c, nb, (uint32_t)sA, w, S, a, r);
   if (a[0] || a[1]) {
     mcsu::pair_apply<PAR>(c, S, a);
 #pragma unroll
     for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(pl[d] + own) = c[d];
   }
-  if (RECORD) mcsu::pair_record<POT>(P, kc, rec, sA, S, a, r);
+  if (RECORD) mcsu::pair_record<POT>(P, kc, rec, sA, w, S, a, r);
   return (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u);
 }
 
@@ -131,6 +138,9 @@ __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA,
   const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
   const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
   const int xl = row + (q == 0 ? X - 1 : x0 - 1), xr = row + (q == nseg - 1 ? 0 : x0 + 4);
+  const bool zp_on = !P.zfree || k + 1 < Z, zm_on = !P.zfree || k > 0;
+  const int surf = P.zsurf ? (k == 0 ? 0 : (k == Z - 1 ? 1 : -1)) : -1;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float4 c[3], yp[3], ym[3], zp[3], zm[3];
   float ol[3], orr[3];
 #pragma unroll
@@ -138,8 +148,8 @@ __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA,
     c[d] = *reinterpret_cast<const float4 *>(pl[d] + row + x0);
     yp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
     ym[d] = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
-    zp[d] = *reinterpret_cast<const float4 *>(pl[d] + r_kp);
-    zm[d] = *reinterpret_cast<const float4 *>(pl[d] + r_km);
+    zp[d] = zp_on ? *reinterpret_cast<const float4 *>(pl[d] + r_kp) : zero4;
+    zm[d] = zm_on ? *reinterpret_cast<const float4 *>(pl[d] + r_km) : zero4;
     ol[d] = pl[d][xl];
     orr[d] = pl[d][xr];
   }
@@ -158,8 +168,12 @@ __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA,
     }
     double so = 0.0;
 #pragma unroll
-    for (int b = 0; b < 6; ++b) so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+    for (int b = 0; b < 6; ++b) {
+      if ((b == 4 && !zp_on) || (b == 5 && !zm_on)) continue;
+      so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+    }
     float E = (float)so;
+    if (surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
     if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
     const double x = o[0], y = o[1], z = o[2];
     a[0] += (double)E;
@@ -257,7 +271,9 @@ void launch_t(const MCLaunch &L, int colour, int grid, int finalize, int xcd, co
 }  // namespace
 
 bool mc_x4_eligible(const MCLaunch &L) {
-  return L.pt == nullptr && L.P.n_surf == 0 && (L.P.nx % 4) == 0 && L.P.per[0] && L.P.per[1] && L.P.per[2];
+  if ((L.P.nx % 4) != 0 || !L.P.per[0] || !L.P.per[1]) return false;
+  if (L.P.zfree) return true;  // slab geometry, verified against pt and ns at create
+  return L.pt == nullptr && L.P.n_surf == 0 && L.P.per[2];
 }
 
 int mc_x4_slots(const MCDevParams &P) { return (P.nx / 4) * P.ny * P.nz; }

@@ -51,6 +51,13 @@ def make_case(name, rng):
         kw = {"k": (1.48, 1.0, 2.28, 0.0), "elec": (0.0, 0.0, 1.0), "elecA": 1.0, "elecE": 0.45}
     elif name == "ll_holes":
         dims, bound, pot, kw = (6, 6, 6), (1, 1, 1), O.LL, {}
+    elif name == "ll_slab":
+        # LL between homeotropic (k = 0) and planar (k = Nz-1) faces, with a field
+        dims, bound, pot = (16, 8, 10), (1, 1, 0), O.LL
+        surf = [("homeotropic", 3.0, (0.0, 0.0, 1.0)), ("rp", 2.0, (1.0, 0.0, 0.0))]
+        kw = {"A": 1.0, "elec": (0.0, 0.0, 1.0), "elecA": 0.6, "elecE": 0.5}
+    elif name == "ll_freez":
+        dims, bound, pot, kw = (16, 8, 9), (1, 1, 0), O.LL, {}
     else:
         raise KeyError(name)
     nx, ny, nz = dims
@@ -58,7 +65,7 @@ def make_case(name, rng):
     po = O.params(nx, ny, nz, bound=bound, potential=pot, surfaces=surf, **kw)
     pt = np.ones(nt, np.int32)
     ns = None
-    if name.startswith("slab_"):
+    if name.startswith("slab_") or name == "ll_slab":
         ns = np.zeros(3 * nt, np.float32)
         for j in range(ny):
             for i in range(nx):
@@ -317,6 +324,50 @@ def test_segment_kernel_replay(pot, T, field, kernel):
     want /= np.linalg.norm(want, axis=1, keepdims=True)
     assert np.abs(rec["prop"].reshape(-1, 3) - want).max() < 5e-6
     np.testing.assert_allclose(np.linalg.norm(rec["prop"].reshape(-1, 3), axis=1), 1.0, atol=3e-7)
+
+
+SLAB_GEOMETRY = ["slab_" + k for k in ("homeotropic", "rp", "fg", "strong", "homeotropic_ghrl", "rp_ghrl",
+                                        "fg_ghrl", "strong_ghrl")] + ["ll_slab", "ll_freez"]
+
+
+@pytest.mark.parametrize("kernel", ["x4", "slab"])
+@pytest.mark.parametrize("case", SLAB_GEOMETRY)
+def test_fast_kernels_slab_geometry_replay(case, kernel):
+    # x, y periodic, z free, surfaces on the two faces (or none): the segment
+    # and slab kernels drop the missing z bond on each face and add the
+    # surface term there; recorded sweeps must replay exactly in the oracle.
+    rng = np.random.default_rng(zlib.crc32(case.encode()) + 7)
+    po, cfg, pt, ns = make_case(case, rng)
+    nx, ny, nz = po.nx, po.ny, po.nz
+    nt = nx * ny * nz
+    n = unit(rng, nt)
+    ref = n.copy()
+    tiles = {(8, 6, 6): (3, 2), (16, 8, 10): (4, 5), (16, 8, 9): (4, 3)}
+    kind = mclics.KERNEL_X4 if kernel == "x4" else mclics.KERNEL_SLAB
+    with mclics.Lattice(cfg, n, pt, ns) as lat:
+        assert lat.kernel() == mclics.KERNEL_SLAB  # AUTO takes the fast path for this geometry
+        lat.set_kernel(kind)
+        if kernel == "slab":
+            lat.set_slab_tile(*tiles[(nx, ny, nz)])  # tiles meet inside the lattice on y and z
+        for w, T in enumerate((0.9, 0.05, 2.5)):
+            lat.set_T(float(np.float32(T)))
+            av = lat.ang_var()
+            rec = lat.record_sweep()
+            total = 0
+            for colour in (0, 1):
+                cnt, racc, reo, ren = O.pass_inject(po, ref, pt, ns, 8 + colour, np.float32(T), rec["prop"],
+                                                    rec["u2"])
+                sel = in_set(nx, ny, nz, 8 + colour)
+                assert np.array_equal(rec["accept"][sel], racc[sel]), (case, w, colour)
+                assert np.array_equal(rec["e_old"][sel], reo[sel]), (case, w, colour)
+                assert np.array_equal(rec["e_new"][sel], ren[sel]), (case, w, colour)
+                total += cnt
+            assert np.array_equal(lat.directors(), ref), (case, w)
+            assert lat.acceptance() == (total, w + 1)
+            assert lat.ang_var() == O.lib().orc_adapt_ang_var(av, total, nt)
+        E, S, q = lat.measure()
+        assert E == pytest.approx(O.energy_mean(po, ref, pt, ns, gpu=1), rel=1e-12, abs=1e-12)
+        np.testing.assert_allclose(q, O.qtensor(ref, pt), rtol=1e-11, atol=1e-13)
 
 
 def test_segment_kernel_selection_rules():
