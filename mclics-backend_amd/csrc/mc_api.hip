@@ -205,7 +205,12 @@ int select_kernel(mc_handle *h, int kind) {
   int want = MC_KERNEL_GENERIC;
   MCSlabShape s;
   const bool slab_ok = fast_eligible(h) && (h->slab.ty > 0 || slab_default(h, &s));
-  if (kind == MC_KERNEL_SLAB || (kind == MC_KERNEL_AUTO && slab_ok)) want = MC_KERNEL_SLAB;
+  // AUTO: the slab kernel for Lebwohl-Lasher, whose screened update leaves
+  // the sweep bound by bytes and issue; GHRL and pear evaluate every move
+  // with the exact FP32/FP64 arithmetic, and the segment kernel's higher
+  // occupancy and lack of halo work serve that better (mc_autotune times both).
+  if (kind == MC_KERNEL_SLAB || (kind == MC_KERNEL_AUTO && slab_ok && h->P.potential == MCD_LL))
+    want = MC_KERNEL_SLAB;
   else if (kind == MC_KERNEL_X4 || (kind == MC_KERNEL_AUTO && fast_eligible(h))) want = MC_KERNEL_X4;
   if (want == MC_KERNEL_SLAB) {
     if (!slab_ok) return fail(MC_EINVAL, "the slab kernel needs a checkerboard order and an all-active lattice, "
@@ -733,17 +738,50 @@ static int autotune_slab(mc_handle *h, int *grid, int *xcd, float *best_us) {
   return MC_OK;
 }
 
+static int autotune_x4(mc_handle *h, int *grid, int *xcd, float *best_us);
+
 int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us) {
   int rc = check_handle(h);
   if (rc) return rc;
+  if (h->kernel == MC_KERNEL_AUTO && (use_slab(h) || use_x4(h))) {
+    // Time both fast kernels at their best shapes and keep the faster.
+    int gs = 0, xs = 0, gx = 0, xx = 0;
+    float us_s = 1e30f, us_x = 1e30f;
+    bool have_slab = select_kernel(h, MC_KERNEL_SLAB) == MC_OK;
+    if (have_slab) {
+      rc = autotune_slab(h, &gs, &xs, &us_s);
+      if (rc) return rc;
+    }
+    rc = select_kernel(h, MC_KERNEL_X4);
+    if (rc) return rc;
+    rc = autotune_x4(h, &gx, &xx, &us_x);
+    if (rc) return rc;
+    const bool slab_wins = have_slab && us_s <= us_x;
+    if (slab_wins) {
+      rc = select_kernel(h, MC_KERNEL_SLAB);
+      if (rc) return rc;
+    }
+    h->kernel = MC_KERNEL_AUTO;
+    MC_HIP(hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    if (grid) *grid = slab_wins ? gs : gx;
+    if (xcd) *xcd = slab_wins ? xs : xx;
+    if (best_us) *best_us = slab_wins ? us_s : us_x;
+    return MC_OK;
+  }
   if (use_slab(h)) return autotune_slab(h, grid, xcd, best_us);
   if (!use_x4(h)) {
     if (grid) *grid = 0;
     if (xcd) *xcd = 0;
     return MC_OK;
   }
+  return autotune_x4(h, grid, xcd, best_us);
+}
+
+static int autotune_x4(mc_handle *h, int *grid, int *xcd, float *best_us) {
   // Candidates run on a scratch copy of the lattice and state, so tuning
   // leaves the chain untouched.
+  int rc = MC_OK;
   const size_t nt = (size_t)h->P.nt;
   float *scratch = nullptr;
   MCDevState *sst = nullptr;

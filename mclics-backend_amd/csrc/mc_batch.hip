@@ -199,8 +199,10 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target) {
 
 // Resolves the requested kernel: the slab when every replica has a tile
 // shape and the second plane set can be allocated, else the segment kernel.
+// AUTO takes the slab for Lebwohl-Lasher only (GHRL and pear evaluate every
+// move exactly, which the segment kernel's occupancy serves better).
 int select_kernel(mc_batch *b, int kind) {
-  bool want = kind == MC_KERNEL_SLAB || kind == MC_KERNEL_AUTO;
+  bool want = kind == MC_KERNEL_SLAB || (kind == MC_KERNEL_AUTO && b->potential == MCD_LL);
   if (want && b->slab_budget == 0) {
     std::vector<int4> wg;
     std::vector<int2> geo;

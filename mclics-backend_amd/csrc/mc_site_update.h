@@ -14,14 +14,15 @@
 // only, so a workgroup that recomputes a neighbour's update (the slab kernel's
 // halo) reaches the same proposal and decision.
 //
-// Accept decision.  For Lebwohl-Lasher the energy difference is first formed
-// in FP32 with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2), whose distance from the
-// reference's own FP32/FP64 arithmetic is bounded by 2^-14*(1.5|A| + |field|)
-// (six bonds, a few roundings each, with a 4x margin).  A hardware exp2 then
-// decides every move whose uniform lies outside that error band around
-// exp(-dE/T); moves inside the band (about 1e-4 of them at T ~ 1) are decided
-// with the exact reference arithmetic, so every decision equals the one the
-// CPU path makes.  GHRL and pear always use the exact arithmetic.
+// Accept decision.  The energy difference is first formed in FP32: for
+// Lebwohl-Lasher with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2); for GHRL and
+// pear bond by bond with the reference's FP64 steps in FP32.  Its distance
+// from the reference's own FP32/FP64 arithmetic has a proven bound (below),
+// and the band eps around exp(-dE/T) keeps a several-fold margin over it.  A
+// hardware exp2 decides every move whose uniform lies outside the band; moves
+// inside it (a few in 10^4 to 10^2 at T ~ 1) are decided with the exact
+// reference arithmetic, so every decision equals the one the CPU path makes.
+// Surface sites are always decided exactly.
 #pragma once
 #include "mc_kernels.h"
 
@@ -123,6 +124,17 @@ struct PassConst {
   float invT, epsT;  // 1/T and eps/T for the FP32 screen
 };
 
+// Bound on one bond's magnitude from each term's coefficient (unit vectors):
+//   GHRL: |E1 (rho P2 + lambda)| <= 2 (|rho| + |lambda|), |mu (a_i a_j n_ij - 1/9)| <= 1.12 |mu|,
+//         |nu P2| <= |nu|, |sigma cross| <= |sigma|;
+//   pear: |A n^2 + B1 n| <= |A| + |B1|, |0.25 C (1 + 2n + n^2) d| <= 0.25 |C| * 4 * 2.
+__device__ __forceinline__ float bond_bound(const MCDevParams &P) {
+  if (P.potential == MCD_GHRL)
+    return 2.0f * (fabsf(P.rho) + fabsf(P.lambda)) + 1.12f * fabsf(P.mu) + fabsf(P.nu) + fabsf(P.sigma);
+  if (P.potential == MCD_PEAR) return fabsf(P.A) + fabsf(P.B1) + 2.0f * fabsf(P.C);
+  return 1.5f * fabsf(P.A);
+}
+
 __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDevState *st) {
   PassConst k;
   k.T = st->T;
@@ -132,7 +144,11 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
   k.cA = (double)(-P.A) * 1.5;
   k.cAf = (float)k.cA;
   const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
-  k.eps = 6.1035156e-5f * (fabsf(k.cAf) + fabsf(P.elecA) * k.elecE * k.elecE * e2);  // 2^-14 * (...)
+  const float F = fabsf(P.elecA) * k.elecE * k.elecE * e2;
+  // 2^-14 * (...): LL with the bound 1.5|A| itself (screen_ll); GHRL and pear
+  // with twice the bond bound (screen_bonds)
+  const float M = bond_bound(P);
+  k.eps = 6.1035156e-5f * ((P.potential == MCD_LL ? M : 2.0f * M) + F);
   k.invT = 1.0f / k.T;
   k.epsT = k.eps / k.T;
   return k;
@@ -160,6 +176,63 @@ __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &
     acc = fmaf(e1, e2, acc);
   }
   float dE = k.cAf * acc;
+  if (P.elecA != 0.0f)
+    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
+  return fast_decide(dE, k.invT, S.u, k.epsT);
+}
+
+// FP32 screen of one GHRL or pear move: each bond of the reference
+// (src/potential.cpp:203-231) with every FP64 step done in FP32.  n.r, m.r
+// and (n x m).r are read off as signed components (r is a lattice axis), the
+// rest is the reference's FP32 arithmetic.  Error (u = 2^-24, unit vectors,
+// M = bond_bound): the reference's n_ij, P2, cross, nu P2 and sigma sg cross
+// are the same FP32 operations here, so the sign of n_ij agrees and the
+// chiral term's jump cannot differ; E1 in FP32 is off by <= 8u, so E1 (rho P2
+// + lambda) by <= 9u (|rho| + |lambda|); mu (a a n - 1/9) by <= 2.3u |mu|;
+// the three FP32 additions of a bond by <= 2u M; pear's f - 0.25 C ps by
+// <= 3u M.  So each bond is within 12u M of the reference's (FP64 sum rounded
+// once), twelve of them within 144u M, the FP32 accumulation of the twelve
+// differences adds <= 48u M, the reference's own roundings of its two site
+// energies and their difference <= 12u M, the field terms (the same FP32
+// operations) <= 4u F: <= 210u M + 4u F against eps = 2^-14 (2M + F) =
+// 2048u M + 1024u F, a tenfold margin.
+template <int POT>
+__device__ __forceinline__ float bond32(const MCDevParams &P, int b, const float n[3], const float m[3]) {
+  const int a = b >> 1;
+  const float sg = (b & 1) ? -1.0f : 1.0f;
+  const float nij = mcd_dot(n[0], n[1], n[2], m[0], m[1], m[2]);
+  if (POT == MCD_GHRL) {
+    const float ai = sg * n[a], aj = sg * m[a];
+    const float pij = 1.5f * nij * nij - 0.5f;
+    const float ca = a == 0 ? n[2] * m[1] - n[1] * m[2] : (a == 1 ? n[0] * m[2] - n[2] * m[0] : n[1] * m[0] - n[0] * m[1]);
+    const float cross = sg * ca;
+    float E1 = 1.5f * ai * ai;
+    E1 = E1 + 1.5f * aj * aj;
+    E1 = E1 - 1.0f;
+    const float t1 = E1 * (P.rho * pij + P.lambda);
+    const float t2 = P.mu * (ai * aj * nij - 0.111111112f);
+    const float ss = nij > 0 ? 1.0f : -1.0f;
+    return ((t1 + t2) + P.nu * pij) + P.sigma * ss * cross;
+  }
+  float aa = 1.0f + 2.0f * nij;
+  aa = aa + nij * nij;
+  const float d = sg * (m[a] - n[a]);
+  const float ps = aa * d;
+  float f = -P.A * nij * nij;
+  f = f - P.B1 * nij;
+  return f - (P.C * 0.25f) * ps;
+}
+
+template <int POT>
+__device__ __forceinline__ int screen_bonds(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
+  if (w.surf >= 0) return -1;
+  float acc = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    if (!((w.on >> b) & 1u)) continue;
+    acc += bond32<POT>(P, b, S.q, S.m[b]) - bond32<POT>(P, b, S.o, S.m[b]);
+  }
+  float dE = acc;
   if (P.elecA != 0.0f)
     dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
   return fast_decide(dE, k.invT, S.u, k.epsT);
@@ -202,30 +275,28 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   propose(S[1].o, k.av, r.z, ax1, S[1].q);
   S[0].u = mcd_u01(r.y);
   S[1].u = mcd_u01(r.w);
+  // Screen both sites, then one shared (rarely executed) exact evaluation for
+  // whichever sites fell inside the error band.
+  int d0, d1;
   if (POT == MCD_LL) {
-    // Screen both sites, then one shared (rarely executed) exact evaluation
-    // for whichever sites fell inside the error band.
-    const int d0 = screen_ll(P, k, S[0], w), d1 = screen_ll(P, k, S[1], w);
-    a[0] = d0 == 1;
-    a[1] = d1 == 1;
-    if (d0 < 0 || d1 < 0) {
-#pragma unroll 1
-      for (int t = 0; t < 2; ++t) {
-        if ((t ? d1 : d0) >= 0) continue;
-        const Site &X = t ? S[1] : S[0];
-        float Eo, En;
-        exact_energies<POT>(P, k.cA, k.elecE, X, w, Eo, En);
-        const bool at = mcd_accept(En, Eo, k.T, X.u);
-        if (t) a[1] = at; else a[0] = at;
-      }
-    }
-    return;
+    d0 = screen_ll(P, k, S[0], w);
+    d1 = screen_ll(P, k, S[1], w);
+  } else {
+    d0 = screen_bonds<POT>(P, k, S[0], w);
+    d1 = screen_bonds<POT>(P, k, S[1], w);
   }
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {  // GHRL and pear: the exact arithmetic throughout
-    float Eo, En;
-    exact_energies<POT>(P, k.cA, k.elecE, S[t], w, Eo, En);
-    a[t] = mcd_accept(En, Eo, k.T, S[t].u);
+  a[0] = d0 == 1;
+  a[1] = d1 == 1;
+  if (d0 < 0 || d1 < 0) {
+#pragma unroll 1
+    for (int t = 0; t < 2; ++t) {
+      if ((t ? d1 : d0) >= 0) continue;
+      const Site &X = t ? S[1] : S[0];
+      float Eo, En;
+      exact_energies<POT>(P, k.cA, k.elecE, X, w, Eo, En);
+      const bool at = mcd_accept(En, Eo, k.T, X.u);
+      if (t) a[1] = at; else a[0] = at;
+    }
   }
 }
 

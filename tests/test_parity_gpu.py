@@ -345,7 +345,8 @@ def test_fast_kernels_slab_geometry_replay(case, kernel):
     tiles = {(8, 6, 6): (3, 2), (16, 8, 10): (4, 5), (16, 8, 9): (4, 3)}
     kind = mclics.KERNEL_X4 if kernel == "x4" else mclics.KERNEL_SLAB
     with mclics.Lattice(cfg, n, pt, ns) as lat:
-        assert lat.kernel() == mclics.KERNEL_SLAB  # AUTO takes the fast path for this geometry
+        # AUTO takes a fast kernel for this geometry: the slab for LL, the segment kernel otherwise
+        assert lat.kernel() == (mclics.KERNEL_SLAB if po.potential == O.LL else mclics.KERNEL_X4)
         lat.set_kernel(kind)
         if kernel == "slab":
             lat.set_slab_tile(*tiles[(nx, ny, nz)])  # tiles meet inside the lattice on y and z
@@ -457,11 +458,22 @@ def test_slab_autotune_leaves_chain_and_keeps_parity():
         assert a.kernel() == mclics.KERNEL_SLAB
         a.sweeps(3)
         b.sweeps(3)
+        a.set_kernel(mclics.KERNEL_SLAB)  # an explicit kernel: only its shapes are tuned
         grid, xcd, us = a.autotune()
         ty, cz, x = a.slab_tile()
         assert grid == (ny // ty) * (nz // cz) and x == xcd and us > 0
         a.sweeps(4, graph=True)
         b.sweeps(4)
+        a.sync()
+        b.sync()
+        assert np.array_equal(a.directors(), b.directors())
+        assert a.acceptance() == b.acceptance() and a.ang_var() == b.ang_var()
+        # AUTO: both fast kernels are timed and the faster one kept; the chain goes on unchanged
+        a.set_kernel(mclics.KERNEL_AUTO)
+        grid, xcd, us = a.autotune()
+        assert a.kernel() in (mclics.KERNEL_SLAB, mclics.KERNEL_X4) and grid > 0 and us > 0
+        a.sweeps(5, graph=True)
+        b.sweeps(5)
         a.sync()
         b.sync()
         assert np.array_equal(a.directors(), b.directors())
