@@ -146,6 +146,42 @@ def batch_leg(n_sweeps=400):
             "S_mean": float(np.mean(S)), "E_mean": float(np.mean(E))}
 
 
+def slab_geometry_leg(n_sweeps=400):
+    """Config C3: 128x128x64 GHRL (k11, k22, k33 = 1.0, 0.7, 1.3; p0 = 0)
+    between homeotropic GHRL anchoring faces (W = 5), field along z (elecA = 1,
+    elecE = 0.5), T = 0.5, periodic x/y and free z, homogeneous start along x.
+    Sweeps are timed after autotuning the fast kernels; 25 B per site-update
+    (the 24 B of the directors plus the 1 B point type of the slab layout)."""
+    import mclics
+    from mclics.ic import homogeneous_ic
+    nx, ny, nz = 128, 128, 64
+    nt = nx * ny * nz
+    k11, k22, k33 = 1.0, 0.7, 1.3
+    S = abs(9.0 / (k11 - 3 * k22 - k33))  # src/io.cpp:282-288, p0 = 0
+    ghrl = (S / 9 * (k11 - k33), S / 9 * (2 * k11 - 3 * k22 + k33), S * (k22 - k11), S / 9 * (k11 - 3 * k22 - k33), 0.0)
+    pt = np.ones(nt, np.int32)
+    ns = np.zeros(3 * nt, np.float32)
+    pt[: nx * ny] = 2
+    pt[nx * ny * (nz - 1):] = 3
+    ns[2: 3 * nx * ny: 3] = -1.0
+    ns[3 * nx * ny * (nz - 1) + 2::3] = 1.0
+    surf = [mclics.Surface("homeotropic_ghrl", 5.0, (0.0, 0.0, 1.0))] * 2
+    cfg = mclics.Config(nx, ny, nz, bound=(mclics.PERIODIC, mclics.PERIODIC, mclics.FREE), potential=mclics.GHRL,
+                        ghrl=[float(np.float32(v)) for v in ghrl], elec=(0.0, 0.0, 1.0), elecA=1.0, elecE=0.5,
+                        surfaces=surf, T=0.5, seed=7)
+    with mclics.Lattice(cfg, homogeneous_ic(nx, ny, nz, 90.0, 0.0), pt=pt, ns_aos=ns) as lat:
+        lat.autotune()
+        lat.sweeps(20, graph=True)
+        ms, _ = lat.time_sweeps(n_sweeps, graph=True)
+        kernel = {mclics.KERNEL_SLAB: "k_sweep_slab", mclics.KERNEL_X4: "k_sweep_x4"}.get(lat.kernel(), "k_sweep_pass")
+        E, S_, _ = lat.measure()
+    rate = n_sweeps * nt / (ms / 1e3)
+    return {"workload": "GHRL 128x128x64, periodic x/y, free z, homeotropic_ghrl faces W=5, field z elecA=1 "
+                        "elecE=0.5, T=0.5", "sites": nt, "sweeps": n_sweeps, "us_per_sweep": ms * 1e3 / n_sweeps,
+            "kernel": kernel, "site_updates_per_s": rate,
+            "roofline_frac_25B": rate * 25 / (HBM_PEAK_GBS * 1e9), "E": E, "S": S_}
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` at 256^3 from the committed byte-counter
     summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the copy
@@ -184,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
+    ap.add_argument("--no-slab-geometry", action="store_true", help="skip the GHRL slab-geometry leg")
     ap.add_argument("--learned-sweeps", type=int, default=30000,
                     help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
@@ -271,6 +308,9 @@ def main():
     if not args.no_batch:
         log("bench: batched-replica leg (64 replicas, 20.6M sites)")
         result["batched_replicas"] = batch_leg()
+    if not args.no_slab_geometry:
+        log("bench: slab-geometry leg (GHRL 128x128x64, anchoring faces, field)")
+        result["slab_geometry"] = slab_geometry_leg()
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
         result["learned_proposal"] = learned_leg(args.learned_sweeps)

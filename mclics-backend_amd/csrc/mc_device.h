@@ -255,7 +255,7 @@ __device__ __forceinline__ float mcd_ghrl_surface_form(const MCDevParams &P, flo
 
 // Surface terms of the CPU anchoring classes (src/anchoring_homeotropic.cpp,
 // src/anchoring_rp.cpp, src/anchoring_fg.cpp, src/anchoring_strong.cpp).
-__device__ __noinline__ float mcd_surface(const MCDevParams &P, int surf, float n0, float n1, float n2, float s0,
+__device__ __forceinline__ float mcd_surface(const MCDevParams &P, int surf, float n0, float n1, float n2, float s0,
                                           float s1, float s2) {
   const float W = P.surf_W[surf];
   const float d0 = P.surf_dir[surf][0], d1 = P.surf_dir[surf][1], d2 = P.surf_dir[surf][2];

@@ -65,20 +65,22 @@ constexpr Where kBulk{0x3Fu, -1};
 
 // Site energies in the order of the reference's site energy
 // (src/geometry.cpp:19-72): bonds summed in FP64 and rounded, then the
-// surface term, then the field.
-template <int POT>
+// surface term, then the field.  GEO: the lattice may have free z faces and
+// surfaces (w matters); without it w is the bulk one and the code is the
+// periodic lattice's alone.
+template <int POT, bool GEO>
 __device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, float elecE, const Site &S,
                                                const Where &w, float &Eo, float &En) {
   double so = 0.0, sn = 0.0;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
-    if (!((w.on >> b) & 1u)) continue;
+    if (GEO && !((w.on >> b) & 1u)) continue;
     so += (double)mcd_bond<POT>(P, cA, b, S.o[0], S.o[1], S.o[2], S.m[b][0], S.m[b][1], S.m[b][2]);
     sn += (double)mcd_bond<POT>(P, cA, b, S.q[0], S.q[1], S.q[2], S.m[b][0], S.m[b][1], S.m[b][2]);
   }
   Eo = (float)so;
   En = (float)sn;
-  if (w.surf >= 0) {
+  if (GEO && w.surf >= 0) {
     const float sz = w.surf ? 1.0f : -1.0f;
     Eo += mcd_surface(P, w.surf, S.o[0], S.o[1], S.o[2], 0.0f, 0.0f, sz);
     En += mcd_surface(P, w.surf, S.q[0], S.q[1], S.q[2], 0.0f, 0.0f, sz);
@@ -164,8 +166,9 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
 // eps = 2^-14 * 1.5|A| = 1024u * 1.5|A|.
 // An absent neighbour is a zero vector here, whose bond term is exactly 0;
 // a surface site is always decided exactly.
+template <bool GEO>
 __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
-  if (w.surf >= 0) return -1;
+  if (GEO && w.surf >= 0) return -1;
   const float d0 = S.q[0] - S.o[0], d1 = S.q[1] - S.o[1], d2 = S.q[2] - S.o[2];
   const float s0 = S.q[0] + S.o[0], s1 = S.q[1] + S.o[1], s2 = S.q[2] + S.o[2];
   float acc = 0.0f;
@@ -223,13 +226,13 @@ __device__ __forceinline__ float bond32(const MCDevParams &P, int b, const float
   return f - (P.C * 0.25f) * ps;
 }
 
-template <int POT>
+template <int POT, bool GEO>
 __device__ __forceinline__ int screen_bonds(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
-  if (w.surf >= 0) return -1;
+  if (GEO && w.surf >= 0) return -1;
   float acc = 0.0f;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
-    if (!((w.on >> b) & 1u)) continue;
+    if (GEO && !((w.on >> b) & 1u)) continue;
     acc += bond32<POT>(P, b, S.q, S.m[b]) - bond32<POT>(P, b, S.o, S.m[b]);
   }
   float dE = acc;
@@ -250,7 +253,7 @@ struct PairNbrs {
 // index sA; c is the segment.  Fills S (for recording) and a.
 // An absent z neighbour (free face) must arrive as a zero vector with its
 // bit clear in w.on.
-template <int POT, int PAR>
+template <int POT, int PAR, bool GEO>
 __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassConst &k, const float4 c[3],
                                             const PairNbrs &nb, uint32_t sA, const Where &w, Site S[2], bool a[2],
                                             MCU4 &r) {
@@ -279,11 +282,11 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   // whichever sites fell inside the error band.
   int d0, d1;
   if (POT == MCD_LL) {
-    d0 = screen_ll(P, k, S[0], w);
-    d1 = screen_ll(P, k, S[1], w);
+    d0 = screen_ll<GEO>(P, k, S[0], w);
+    d1 = screen_ll<GEO>(P, k, S[1], w);
   } else {
-    d0 = screen_bonds<POT>(P, k, S[0], w);
-    d1 = screen_bonds<POT>(P, k, S[1], w);
+    d0 = screen_bonds<POT, GEO>(P, k, S[0], w);
+    d1 = screen_bonds<POT, GEO>(P, k, S[1], w);
   }
   a[0] = d0 == 1;
   a[1] = d1 == 1;
@@ -293,7 +296,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
       if ((t ? d1 : d0) >= 0) continue;
       const Site &X = t ? S[1] : S[0];
       float Eo, En;
-      exact_energies<POT>(P, k.cA, k.elecE, X, w, Eo, En);
+      exact_energies<POT, GEO>(P, k.cA, k.elecE, X, w, Eo, En);
       const bool at = mcd_accept(En, Eo, k.T, X.u);
       if (t) a[1] = at; else a[0] = at;
     }
@@ -315,7 +318,7 @@ __device__ __forceinline__ void pair_apply(float4 c[3], const Site S[2], const b
 }
 
 // Record of both sites of a pair (the parity replay reads it back).
-template <int POT>
+template <int POT, bool GEO>
 __device__ __forceinline__ void pair_record(const MCDevParams &P, const PassConst &k, const MCRecord &rec, int sA,
                                             const Where &w, const Site S[2], const bool a[2], const MCU4 &r) {
   const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
@@ -324,7 +327,7 @@ __device__ __forceinline__ void pair_record(const MCDevParams &P, const PassCons
   for (int t = 0; t < 2; ++t) {
     const int s = sA + 2 * t;
     float Eo, En;
-    exact_energies<POT>(P, k.cA, k.elecE, S[t], w, Eo, En);
+    exact_energies<POT, GEO>(P, k.cA, k.elecE, S[t], w, Eo, En);
     rec.u0[s] = mcd_u01(t ? r.z : r.x);
     rec.u1[s] = (float)(t ? ax1 : ax0) * (1.0f / 65536.0f);
     rec.u2[s] = S[t].u;

@@ -67,13 +67,19 @@ struct LaneOff {
 // Plane index z in [-2, Z+2) wrapped onto the periodic lattice.
 __device__ __forceinline__ int wrap_z(const SlabGeom &g, int z) { return z < 0 ? z + g.Z : (z >= g.Z ? z - g.Z : z); }
 
-// Plane z exists: always for periodic z, inside [0, Z) for a free z.
-__device__ __forceinline__ bool has_plane(const SlabGeom &g, int z) { return !g.zfree || (z >= 0 && z < g.Z); }
+// Plane z exists: always for periodic z, inside [0, Z) for a free z.  GEO
+// (the lattice may have free z faces and surfaces) is a template parameter
+// throughout, so the periodic lattice's code carries none of it.
+template <bool GEO>
+__device__ __forceinline__ bool has_plane(const SlabGeom &g, int z) {
+  return !GEO || !g.zfree || (z >= 0 && z < g.Z);
+}
 
 // A plane beyond a free face reads as zero vectors.
+template <bool GEO>
 __device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], const SlabGeom &g, const LaneOff &L,
                                          int z, float4 v[3]) {
-  if (!has_plane(g, z)) {
+  if (!has_plane<GEO>(g, z)) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) v[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
@@ -84,9 +90,10 @@ __device__ __forceinline__ void load_seg(const float *__restrict__ const in[3], 
 }
 
 // Neighbours and surface of plane z (uniform over the workgroup).
+template <bool GEO>
 __device__ __forceinline__ mcsu::Where where_z(const SlabGeom &g, int z) {
   mcsu::Where w = mcsu::kBulk;
-  if (g.zfree) {
+  if (GEO && g.zfree) {
     if (z + 1 >= g.Z) w.on &= ~(1u << 4);
     if (z <= 0) w.on &= ~(1u << 5);
     if (g.zsurf) w.surf = z == 0 ? 0 : (z == g.Z - 1 ? 1 : -1);
@@ -115,7 +122,7 @@ __device__ __forceinline__ void put_half(float *slot, const LaneOff &L, int par,
 // own: the segment (updated in place); zm/zp: the segments of planes z-1 and
 // z+1 in this column; oth: the half-row slot holding the other colour of
 // plane z.  Returns the accepted count; records when asked.
-template <int POT, bool RECORD, int PAR>
+template <int POT, bool RECORD, int PAR, bool GEO>
 __device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
                                                 const LaneOff &L, float4 own[3], const float4 zm[3],
                                                 const float4 zp[3], const float *oth, int z, bool counted,
@@ -132,24 +139,24 @@ __device__ __forceinline__ uint32_t update_pair(const MCDevParams &P, const mcsu
     nb.ov[d] = sl[mo];
   }
   const int sA = g.XY * wrap_z(g, z) + L.rowbase + PAR;
-  const mcsu::Where w = where_z(g, z);
+  const mcsu::Where w = where_z<GEO>(g, z);
   mcsu::Site S[2];
   bool a[2];
   MCU4 rr;
-  mcsu::pair_update<POT, PAR>(P, kc, own, nb, (uint32_t)sA, w, S, a, rr);
+  mcsu::pair_update<POT, PAR, GEO>(P, kc, own, nb, (uint32_t)sA, w, S, a, rr);
   mcsu::pair_apply<PAR>(own, S, a);
-  if (RECORD && counted) mcsu::pair_record<POT>(P, kc, rec, sA, w, S, a, rr);
+  if (RECORD && counted) mcsu::pair_record<POT, GEO>(P, kc, rec, sA, w, S, a, rr);
   return counted ? (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u) : 0u;
 }
 
-template <int POT, bool RECORD>
+template <int POT, bool RECORD, bool GEO>
 __device__ __forceinline__ uint32_t update(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
                                            const LaneOff &L, int colour, float4 own[3], const float4 zm[3],
                                            const float4 zp[3], const float *oth, int y, int z, bool counted,
                                            const MCRecord &rec) {
   if (((y + z + colour) & 1) == 0)
-    return update_pair<POT, RECORD, 0>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
-  return update_pair<POT, RECORD, 1>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
+    return update_pair<POT, RECORD, 0, GEO>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
+  return update_pair<POT, RECORD, 1, GEO>(P, kc, g, L, own, zm, zp, oth, z, counted, rec);
 }
 
 __device__ __forceinline__ void copy3(float4 dst[3], const float4 src[3]) {
@@ -190,7 +197,7 @@ __device__ __forceinline__ SlabGeom slab_geom(const MCDevParams &P, int TY, int 
 // One tile (rows y0.., planes z0..) of one lattice: the whole sweep of the
 // tile described at the top of this file.  Returns this thread's accepted
 // count (owned sites only); every thread of the workgroup must call it.
-template <int POT, bool RECORD>
+template <int POT, bool RECORD, bool GEO>
 __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
                                               const float *const in[3], float *const out[3], int y0, int z0,
                                               const MCRecord &rec) {
@@ -220,18 +227,19 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
 
   // Prologue: planes z0-2 .. z0+2; red updates of z0-1 and z0.
   if (active) {
-    load_seg(in, g, L, z0 - 2, Q);
-    load_seg(in, g, L, z0 - 1, P0);
-    load_seg(in, g, L, z0, P1);
-    load_seg(in, g, L, z0 + 1, P2);
-    load_seg(in, g, L, z0 + 2, P3);
+    load_seg<GEO>(in, g, L, z0 - 2, Q);
+    load_seg<GEO>(in, g, L, z0 - 1, P0);
+    load_seg<GEO>(in, g, L, z0, P1);
+    load_seg<GEO>(in, g, L, z0 + 1, P2);
+    load_seg<GEO>(in, g, L, z0 + 2, P3);
     put_half(Bbuf((z0 - 1) & 1), L, ((y + z0 - 1 + 1) & 1), P0);
     put_half(Bbuf(z0 & 1), L, ((y + z0 + 1) & 1), P1);
   }
   __syncthreads();
   if (active && red_row) {
-    if (has_plane(g, z0 - 1)) update<POT, false>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
-    acc += update<POT, RECORD>(P, kc, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
+    if (has_plane<GEO>(g, z0 - 1))
+      update<POT, false, GEO>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
+    acc += update<POT, RECORD, GEO>(P, kc, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
     put_half(Rbuf(z0 & 1), L, ((y + z0) & 1), P1);
   }
   __syncthreads();
@@ -241,15 +249,15 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   for (int it = 0; it < CZ; ++it) {
     const int p = z0 + it;
     if (active) {
-      if (it <= CZ - 2) load_seg(in, g, L, p + 3, P4);
+      if (it <= CZ - 2) load_seg<GEO>(in, g, L, p + 3, P4);
       if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
-      if (red_row && has_plane(g, p + 1)) {
-        acc += update<POT, RECORD>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
+      if (red_row && has_plane<GEO>(g, p + 1)) {
+        acc += update<POT, RECORD, GEO>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
                                    owned_row && it <= CZ - 2, rec);
         if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);
       }
       if (owned_row) {
-        acc += update<POT, RECORD>(P, kc, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
+        acc += update<POT, RECORD, GEO>(P, kc, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
         const int off = g.XY * p + L.rowbase;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {  // streaming stores: nothing reads them before the next launch
@@ -268,7 +276,7 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   return acc;
 }
 
-template <int POT, bool RECORD>
+template <int POT, bool RECORD, bool GEO>
 __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const float *__restrict__ inx,
                                                             const float *__restrict__ iny,
                                                             const float *__restrict__ inz, float *__restrict__ outx,
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
   int tile = blockIdx.x;
   if (xcd && (ntiles & 7) == 0) tile = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
   const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
-  const uint32_t acc = slab_tile<POT, RECORD>(P, kc, g, in, out, y0, z0, rec);
+  const uint32_t acc = slab_tile<POT, RECORD, GEO>(P, kc, g, in, out, y0, z0, rec);
   const uint32_t bsum = slab_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, true);
 }
@@ -309,7 +317,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *_
   const float *const in[3] = {inx + off, iny + off, inz + off};
   float *const out[3] = {outx + off, outy + off, outz + off};
   const mcsu::PassConst kc = mcsu::pass_const(P, st);
-  const uint32_t acc = slab_tile<POT, false>(P, kc, g, in, out, d.y, d.z, MCRecord{});
+  const uint32_t acc = slab_tile<POT, false, false>(P, kc, g, in, out, d.y, d.z, MCRecord{});
   const uint32_t bsum = slab_block_sum(acc);
   if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
 }
@@ -319,12 +327,18 @@ void launch_t(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabSh
   const int threads = (L.P.nx / 4) * (s.ty + 4);
   const int grid = (L.P.ny / s.ty) * (L.P.nz / s.cz);
   const size_t lds = mc_slab_lds_bytes(L.P, s.ty);
-  if (rec)
-    k_sweep_slab<POT, true><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, s.cz,
-                                                              s.xcd, *rec);
-  else
-    k_sweep_slab<POT, false><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, s.cz,
-                                                               s.xcd, MCRecord{});
+  const MCRecord r = rec ? *rec : MCRecord{};
+#define MC_SLAB(REC_, GEO_)                                                                               \
+  k_sweep_slab<POT, REC_, GEO_><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, \
+                                                                  s.cz, s.xcd, r)
+  if (L.P.zfree) {
+    if (rec) MC_SLAB(true, true);
+    else MC_SLAB(false, true);
+  } else {
+    if (rec) MC_SLAB(true, false);
+    else MC_SLAB(false, false);
+  }
+#undef MC_SLAB
 }
 
 }  // namespace
@@ -347,12 +361,18 @@ int mc_slab_prepare(size_t lds_bytes) {
       reinterpret_cast<const void *>(&k_batch_slab<MCD_LL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_GHRL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_PEAR>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, false>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, true>),
   };
   int failed = 0;
   for (const void *f : fns)

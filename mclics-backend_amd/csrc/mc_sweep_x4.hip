@@ -41,7 +41,7 @@ __device__ __forceinline__ void x4_row(int rowi, int nseg, int Y, int Z, int &j,
 
 // One thread's segment update; PAR is the x parity of the first site of the
 // pass colour in this row (sites at 4q+PAR and 4q+PAR+2).
-template <int POT, bool RECORD, int PAR>
+template <int POT, bool RECORD, int PAR, bool GEO>
 __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::PassConst &kc, float *__restrict__ nx,
                                             float *__restrict__ ny, float *__restrict__ nz, int q, int j, int k,
                                             const MCRecord &rec) {
@@ -57,10 +57,10 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
   const int out = PAR == 0 ? row + (q == 0 ? X - 1 : x0 - 1) : row + (q == nseg - 1 ? 0 : x0 + 4);
   // free z faces: the missing neighbour is a zero vector with its bit clear
   mcsu::Where w = mcsu::kBulk;
-  const bool zp_on = !P.zfree || k + 1 < Z, zm_on = !P.zfree || k > 0;
+  const bool zp_on = !GEO || !P.zfree || k + 1 < Z, zm_on = !GEO || !P.zfree || k > 0;
   if (!zp_on) w.on &= ~(1u << 4);
   if (!zm_on) w.on &= ~(1u << 5);
-  if (P.zsurf) w.surf = k == 0 ? 0 : (k == Z - 1 ? 1 : -1);
+  if (GEO && P.zsurf) w.surf = k == 0 ? 0 : (k == Z - 1 ? 1 : -1);
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float *pl[3] = {nx, ny, nz};
   float4 c[3];
@@ -82,17 +82,17 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
   mcsu::Site S[2];
   bool a[2];
   MCU4 r;
-  mcsu::pair_update<POT, PAR>(P, kc, c, nb, (uint32_t)sA, w, S, a, r);
+  mcsu::pair_update<POT, PAR, GEO>(P, kc, c, nb, (uint32_t)sA, w, S, a, r);
   if (a[0] || a[1]) {
     mcsu::pair_apply<PAR>(c, S, a);
 #pragma unroll
     for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(pl[d] + own) = c[d];
   }
-  if (RECORD) mcsu::pair_record<POT>(P, kc, rec, sA, w, S, a, r);
+  if (RECORD) mcsu::pair_record<POT, GEO>(P, kc, rec, sA, w, S, a, r);
   return (a[0] ? 1u : 0u) + (a[1] ? 1u : 0u);
 }
 
-template <int POT, bool RECORD>
+template <int POT, bool RECORD, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__restrict__ nx, float *__restrict__ ny,
                                                      float *__restrict__ nz, MCDevState *st, int colour, int finalize,
                                                      int xcd, MCRecord rec) {
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
     int j, k;
     x4_row(rowi, nseg, P.ny, P.nz, j, k);
     if (((j + k + colour) & 1) == 0)
-      acc += segment<POT, RECORD, 0>(P, kc, nx, ny, nz, q, j, k, rec);
+      acc += segment<POT, RECORD, 0, GEO>(P, kc, nx, ny, nz, q, j, k, rec);
     else
-      acc += segment<POT, RECORD, 1>(P, kc, nx, ny, nz, q, j, k, rec);
+      acc += segment<POT, RECORD, 1, GEO>(P, kc, nx, ny, nz, q, j, k, rec);
   }
   const uint32_t bsum = mcd_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, finalize != 0);
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__res
 // Adds the four sites of segment q of row (j, k) to a[8] = {E, 3xx-1, 3xy,
 // 3xz, 3yy-1, 3yz, 3zz-1, n}: each site energy with the reference arithmetic
 // (FP32 bonds summed in FP64, then rounded to FP32), Q terms in FP64.
-template <int POT>
+template <int POT, bool GEO>
 __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA, float elecE,
                                                 const float *const pl[3], int q, int j, int k, double a[8]) {
   const int X = P.nx, Y = P.ny, Z = P.nz;
@@ -138,8 +138,8 @@ __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA,
   const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
   const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
   const int xl = row + (q == 0 ? X - 1 : x0 - 1), xr = row + (q == nseg - 1 ? 0 : x0 + 4);
-  const bool zp_on = !P.zfree || k + 1 < Z, zm_on = !P.zfree || k > 0;
-  const int surf = P.zsurf ? (k == 0 ? 0 : (k == Z - 1 ? 1 : -1)) : -1;
+  const bool zp_on = !GEO || !P.zfree || k + 1 < Z, zm_on = !GEO || !P.zfree || k > 0;
+  const int surf = GEO && P.zsurf ? (k == 0 ? 0 : (k == Z - 1 ? 1 : -1)) : -1;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float4 c[3], yp[3], ym[3], zp[3], zm[3];
   float ol[3], orr[3];
@@ -173,7 +173,7 @@ __device__ __forceinline__ void measure_segment(const MCDevParams &P, double cA,
       so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
     }
     float E = (float)so;
-    if (surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
+    if (GEO && surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
     if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
     const double x = o[0], y = o[1], z = o[2];
     a[0] += (double)E;
@@ -205,7 +205,7 @@ __device__ __forceinline__ double block_sum8(const double a[8]) {
   return v;
 }
 
-template <int POT>
+template <int POT, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const float *__restrict__ nx,
                                                        const float *__restrict__ ny, const float *__restrict__ nz,
                                                        const MCDevState *st, int xcd,
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_measure_x4(MCDevParams P, const floa
     const int t = tile * kBlock + threadIdx.x;
     if (t >= total) continue;
     const int q = t % nseg, rowi = t / nseg;
-    measure_segment<POT>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
+    measure_segment<POT, GEO>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
   }
   const double v = block_sum8(a);
   if (threadIdx.x < 8) partials[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
@@ -252,20 +252,37 @@ __global__ __launch_bounds__(kBlock) void k_batch_measure_x4(const MCDevParams *
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (t < nseg * P.ny * P.nz) {
       const int q = t % nseg, rowi = t / nseg;
-      measure_segment<POT>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
+      measure_segment<POT, false>(P, cA, elecE, pl, q, rowi % P.ny, rowi / P.ny, a);
     }
     const double v = block_sum8(a);
     if (threadIdx.x < 8) partials[(size_t)tile * 8 + threadIdx.x] = v;
   }
 }
 
+template <int POT, bool GEO>
+void launch_g(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
+  if (rec)
+    k_sweep_x4<POT, true, GEO><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
+                                                               *rec);
+  else
+    k_sweep_x4<POT, false, GEO><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
+                                                                MCRecord{});
+}
+
 template <int POT>
 void launch_t(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
-  if (rec)
-    k_sweep_x4<POT, true><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd, *rec);
+  if (L.P.zfree)
+    launch_g<POT, true>(L, colour, grid, finalize, xcd, rec);
   else
-    k_sweep_x4<POT, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
-                                                          MCRecord{});
+    launch_g<POT, false>(L, colour, grid, finalize, xcd, rec);
+}
+
+template <int POT>
+void launch_measure_t(const MCLaunch &L, int grid, int xcd, double *partials) {
+  if (L.P.zfree)
+    k_measure_x4<POT, true><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+  else
+    k_measure_x4<POT, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
 }
 
 }  // namespace
@@ -298,13 +315,13 @@ void mc_launch_batch_measure_tiles(const MCBatchLaunch &B, int grid, double *til
 void mc_launch_measure_x4(const MCLaunch &L, int grid, int xcd, double *partials, double *out, double *series) {
   switch (L.P.potential) {
     case MCD_GHRL:
-      k_measure_x4<MCD_GHRL><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      launch_measure_t<MCD_GHRL>(L, grid, xcd, partials);
       break;
     case MCD_PEAR:
-      k_measure_x4<MCD_PEAR><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      launch_measure_t<MCD_PEAR>(L, grid, xcd, partials);
       break;
     default:
-      k_measure_x4<MCD_LL><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, xcd, partials);
+      launch_measure_t<MCD_LL>(L, grid, xcd, partials);
       break;
   }
   mc_launch_measure_final(L.stream, partials, grid, out, series);
@@ -359,9 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
       int j, k;
       x4_row(rowi, nseg, P.ny, P.nz, j, k);
       if (((j + k + colour) & 1) == 0)
-        acc = segment<POT, false, 0>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
+        acc = segment<POT, false, 0, false>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
       else
-        acc = segment<POT, false, 1>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
+        acc = segment<POT, false, 1, false>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
     }
     const uint32_t bsum = mcd_block_sum(acc);
     if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
