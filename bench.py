@@ -146,6 +146,73 @@ def batch_leg(n_sweeps=400):
             "S_mean": float(np.mean(S)), "E_mean": float(np.mean(E))}
 
 
+def float_loop(xi, xf, dx):
+    """The reference's float loop, for (x = xi; (int)1e6*sign*(x - xf) >= 0;
+    x += dx) in FP32 with sign = (int)(-dx/|dx|) (src/evolve_thermal.cu:77)."""
+    xs, x = [], np.float32(xi)
+    sign = int(-dx / abs(dx))
+    while np.float32(1000000 * sign) * (x - np.float32(xf)) >= 0:
+        xs.append(float(x))
+        x = np.float32(x + np.float32(dx))
+    return xs
+
+
+def c1_run(run_cpu, mct=500, mcs=200):
+    """Config C1, the reference's own CPU-runnable case: 30^3 LL periodic,
+    random start (seed 1), thermal 1.2 -> 0.0 by -0.02 (61 rows of the float
+    loop), MCT = 500 sweeps then MCS = 200 measured sweeps (E and S after
+    each) per temperature.  The whole run on the GPU; with run_cpu, the same
+    run with the oracle's OpenMP sweep (8 sublattice passes, FP64 E and Q) on
+    the host cores."""
+    import mclics
+    L1 = 30
+    temps = float_loop(1.2, 0.0, -0.02)
+    n0 = mclics.random_ic(L1, L1, L1, seed=1)
+    out = {"workload": "LL 30^3 periodic, thermal 1.2 -> 0.0 by -0.02, MCT=500, MCS=200", "rows": len(temps),
+           "site_updates": len(temps) * (mct + mcs) * L1**3}
+    with mclics.Lattice(mclics.Config(L1, L1, L1, T=temps[0], seed=1), n0) as lat:
+        lat.sweeps(2)
+        lat.sync()
+        t0 = time.perf_counter()
+        rows = []
+        for T in temps:
+            lat.set_T(T)
+            lat.sweeps(mct, graph=True)
+            E, S = lat.measure_series(mcs)
+            rows.append((T, float(S.mean()), float(E.mean())))
+        out["gpu_seconds"] = time.perf_counter() - t0
+        out["kernel"] = {mclics.KERNEL_SLAB: "k_sweep_slab", mclics.KERNEL_X4: "k_sweep_x4"}.get(lat.kernel(),
+                                                                                                  "k_sweep_pass")
+    out["gpu_site_updates_per_s"] = out["site_updates"] / out["gpu_seconds"]
+    out["last_row"] = {"T": rows[-1][0], "S": rows[-1][1], "E": rows[-1][2]}
+    if run_cpu:
+        # the first rows of the same run, as many as fit in about budget_s
+        import _oracle as O
+        budget_s = 12.0
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        p = O.params(L1, L1, L1, A=1.0)
+        n = n0.copy()
+        pt = np.ones(L1**3, np.int32)
+        av, sweep, done = 0.5, 0, 0
+        t0 = time.perf_counter()
+        for T in temps:
+            for w in range(mct + mcs):
+                _, av = O.sweep_rng(p, n, pt, None, np.float32(T), av, seed=1, sweep=sweep, nthreads=threads)
+                sweep += 1
+                if w >= mct:
+                    O.energy_mean(p, n, pt, None, gpu=1, nthreads=threads)
+                    O.eigvals(O.qtensor(n, pt))
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_rows_run"] = done
+        out["cpu_cores"] = threads
+        out["cpu_site_updates_per_s"] = done * (mct + mcs) * L1**3 / dt
+        out["cpu_seconds_full_run"] = out["site_updates"] / out["cpu_site_updates_per_s"]
+    return out
+
+
 def slab_geometry_leg(n_sweeps=400):
     """Config C3: 128x128x64 GHRL (k11, k22, k33 = 1.0, 0.7, 1.3; p0 = 0)
     between homeotropic GHRL anchoring faces (W = 5), field along z (elecA = 1,
@@ -221,6 +288,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
     ap.add_argument("--no-slab-geometry", action="store_true", help="skip the GHRL slab-geometry leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 30^3 thermal run")
     ap.add_argument("--learned-sweeps", type=int, default=30000,
                     help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
@@ -317,6 +385,9 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         result["cpu_baseline"] = cpu_baseline(n0)
+    if not args.no_c1:
+        log("bench: C1 run (30^3 thermal, 61 rows) on the GPU" + ("" if args.no_cpu else " and the host cores"))
+        result["c1_thermal_run"] = c1_run(not args.no_cpu)
     print(json.dumps(result), flush=True)
 
 
