@@ -89,15 +89,18 @@ __device__ __forceinline__ bf16x8 aop_hidden(const __bf16 *W, int ld, int row, i
 // the bf16 bit patterns (v_pk_max_i16, two values per instruction) gives the
 // same bits as rounding the ReLU: a negative input rounds to a value <= 0,
 // whose pattern (sign bit set, or +0) maps to +0.
+// (Conversions two at a time, one v_cvt_pk_bf16_f32 per pair of values,
+// straight into the register pair layout of the operand.)
 typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t cvt2(float x, float y) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{x, y}, bf16x2));
+}
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ bf16x8 pack_hidden(const f4 &a, const f4 &b) {
-  bf16x8 v;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v[e] = (__bf16)a[e];
-    v[4 + e] = (__bf16)b[e];
-  }
-  const s16x8 w = __builtin_elementwise_max(__builtin_bit_cast(s16x8, v), s16x8{0, 0, 0, 0, 0, 0, 0, 0});
+  const u32x4 p = {cvt2(a[0], a[1]), cvt2(a[2], a[3]), cvt2(b[0], b[1]), cvt2(b[2], b[3])};
+  const s16x8 w = __builtin_elementwise_max(__builtin_bit_cast(s16x8, p), s16x8{0, 0, 0, 0, 0, 0, 0, 0});
   return __builtin_bit_cast(bf16x8, w);
 }
 
@@ -239,13 +242,15 @@ __device__ __forceinline__ void write_xrow(__bf16 *Xs, int row, const float x[22
 __device__ __forceinline__ void write_xrow24(__bf16 *Xs, int row, const float x[22], bool valid) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    bf16x8 v;
+    u32x4 v;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = 8 * c + e;
-      v[e] = (__bf16)((valid && k < 22) ? x[k] : 0.0f);
+    for (int e = 0; e < 4; ++e) {
+      const int k = 8 * c + 2 * e;
+      const float x0 = (valid && k < 22) ? x[k] : 0.0f;
+      const float x1 = (valid && k + 1 < 22) ? x[k + 1] : 0.0f;
+      v[e] = cvt2(x0, x1);
     }
-    *reinterpret_cast<bf16x8 *>(Xs + row * 24 + 8 * c) = v;
+    *reinterpret_cast<u32x4 *>(Xs + row * 24 + 8 * c) = v;
   }
 }
 
@@ -722,6 +727,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
   float *Os = Oall[wave];
 
   const float T = st->T, E = st->elecE;
+  const float invT = 1.0f / T;
   const uint32_t sweep = st->sweep;
   const double cA = (double)(-P.A) * 1.5;
   const float cAf = (float)cA;
@@ -747,9 +753,6 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     forward64l(Wl, Bl, Xs, Os);
     float mu[3], lk;
     head(Os + l * 4, b, mu, lk);
-    // The neighbours again (cache hits) rather than 18 registers held live
-    // across the network.
-    gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
     const float kappa = __expf(lk);
     const MCU4 r = mcd_philox(MCU4{(uint32_t)s, sweep, P.stream, 0x4D4Cu}, P.seed_lo, P.seed_hi);
     float q[3];
@@ -780,7 +783,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     const float co = mu[0] * o0 + mu[1] * o1 + mu[2] * o2;
     const float cn = mu[0] * q[0] + mu[1] * q[1] + mu[2] * q[2];
     const float lr = lcosh2(kappa * co) - lcosh2(kappa * cn);
-    const float xlog = -dE / T + lr;
+    const float xlog = -dE * invT + lr;
     const float u = mcd_u01(r.w);
     const bool a = valid && u < __expf(fminf(xlog, 80.0f));
     if (a) {
