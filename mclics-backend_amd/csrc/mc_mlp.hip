@@ -677,9 +677,11 @@ __global__ void k_mlp_dataset(MCDevParams P, const float *__restrict__ nx, const
 __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa, float u_w, float u_phi, bool neg,
                                                  float q[3]) {
   const float v = 1.0f - u_w;  // (0, 1]
-  float w = 1.0f + __logf(v + (1.0f - v) * __expf(-2.0f * kappa)) / kappa;
+  // (hardware reciprocal and square root, 1 ulp: the draw is a sample, and
+  // the Metropolis-Hastings ratio uses the density, not these operations)
+  float w = 1.0f + __logf(v + (1.0f - v) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
   w = fminf(fmaxf(w, -1.0f), 1.0f);
-  const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - w * w));
   // orthonormal e1, e2 perpendicular to mu
   const float ax = fabsf(mu[0]), ay = fabsf(mu[1]), az = fabsf(mu[2]);
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
