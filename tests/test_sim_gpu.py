@@ -142,6 +142,49 @@ W 1 4
         assert np.all(pt3[0] == 2) and np.all(pt3[-1] == 3) and np.all(pt3[1:-1] == 1)
 
 
+def test_thermal_gpu_pear_sphere(tmp_path):
+    """Pear in the sphere geometry (sites outside the droplet inactive, a
+    homeotropic shell): po.dat carries the polarization column P
+    (src/evolve_thermal.cu:72,103-105), |mean director| over all Nt sites
+    (src/parameter_order.cpp:142-155), which must match the snapshot."""
+    text = """Nx 12
+Ny 12
+Nz 12
+MCS 10
+MCT 30
+Ti 1.0
+Tf 0.8
+dT -0.1
+potential pear
+A 1.0
+B1 0.3
+C 0.3
+evol thermalGPU
+ic random
+seed 9
+geometry sphere
+xbound periodic
+ybound periodic
+zbound periodic
+anchoring_type 0 homeotropic
+W 0 2
+"""
+    rc, out, err = run(tmp_path, text)
+    assert rc == 0, (out[-2000:], err[-2000:])
+    hdr, rows = po(tmp_path)
+    assert hdr == "T S varS E varE P"
+    nrow, Ts = O.float_loop(1.0, 0.8, -0.1)
+    assert len(rows) == nrow
+    for r, T in zip(rows, Ts):
+        assert len(r) == 6
+        name = tmp_path / f"director_field_{int(100 * (float(T) + 1e-5))}.csv"
+        _, ijk, n, S, pt = snapshot(name)
+        assert (pt == 0).any() and (pt == 2).any() and (pt == 1).any()
+        # inactive sites keep their directors in the snapshot; P averages all Nt sites
+        P = np.linalg.norm(n.astype(np.float64).mean(axis=0))
+        assert float(r[5]) == pytest.approx(P, abs=2e-5)
+
+
 @pytest.mark.parametrize("mode", ["stepGPU", "quenchGPU"])
 def test_step_and_quench_gpu(tmp_path, mode):
     text = f"""Nx 8
