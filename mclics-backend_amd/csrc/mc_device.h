@@ -341,8 +341,22 @@ __device__ __forceinline__ uint32_t mcd_block_sum(uint32_t v) {
   return t;
 }
 
+// ang_var after a sweep with acc accepted moves of `valid` sites,
+// src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
+__device__ __forceinline__ float mcd_adapt_ang_var(float a, uint32_t acc, int valid) {
+  const double rate = 1.0 * (double)acc / (double)valid;
+  if (rate < 0.5) {
+    a = (float)((double)a * 0.99);
+    if ((double)a < 0.01) a = (float)0.01;
+  } else if (rate > 0.5) {
+    a = (float)((double)a / 0.99);
+    if ((double)a > 1.0) a = (float)((double)a - 0.5);
+  }
+  return a;
+}
+
 // Device-side ang_var adaptation by the last workgroup of a sweep's final
-// pass, src/evolve.cu:378-384 (= src/evolve.cpp:169-178).
+// pass.
 //
 // End of a workgroup's part of a sweep pass, called by every thread after the
 // block sum (thread 0 holds block_acc).  Every pass of one sweep must use the
@@ -376,20 +390,23 @@ __device__ inline void mcd_finish_sweep(const MCDevParams &P, MCDevState *st, ui
   const unsigned long long old2 = atomicAdd(word, (1ull << 32) | (unsigned long long)part);
   if ((uint32_t)(old2 >> 32) != S - 1u) return;
   const uint32_t acc = (uint32_t)old2 + part;
-  if (adapt) {
-    float a = st->ang_var;
-    const double rate = 1.0 * (double)acc / (double)P.valid;
-    if (rate < 0.5) {
-      a = (float)((double)a * 0.99);
-      if ((double)a < 0.01) a = (float)0.01;
-    } else if (rate > 0.5) {
-      a = (float)((double)a / 0.99);
-      if ((double)a > 1.0) a = (float)((double)a - 0.5);
-    }
-    st->ang_var = a;
-  }
+  if (adapt) st->ang_var = mcd_adapt_ang_var(st->ang_var, acc, P.valid);
   st->last_acc = acc;
   *word = 0ull;  // acc_count = blocks_done = 0 for the next sweep
+  st->sweep = st->sweep + 1u;
+}
+
+// End of a batched sweep for one replica, called by thread 0 of each of the
+// replica's nwg workgroups: one 64-bit atomic on its acc_count/blocks_done
+// word; the last arrival adapts the replica's ang_var and advances its sweep.
+__device__ inline void mcd_finish_replica(const MCDevParams &P, MCDevState *st, uint32_t block_acc, uint32_t nwg) {
+  unsigned long long *word = reinterpret_cast<unsigned long long *>(&st->acc_count);
+  const unsigned long long old = atomicAdd(word, (1ull << 32) | (unsigned long long)block_acc);
+  if ((uint32_t)(old >> 32) != nwg - 1u) return;
+  const uint32_t acc = (uint32_t)old + block_acc;
+  st->ang_var = mcd_adapt_ang_var(st->ang_var, acc, P.valid);
+  st->last_acc = acc;
+  *word = 0ull;
   st->sweep = st->sweep + 1u;
 }
 

@@ -75,7 +75,7 @@ void mc_launch_batch_x4(const MCBatchLaunch &B, int colour, int grid, int finali
 // Per-replica ang_var adaptation and sweep advance after a batched sweep.
 void mc_launch_batch_finish(const MCBatchLaunch &B);
 // Batched slab sweep (mc_sweep_slab.hip): one launch per sweep of every
-// replica, reading B.nx.. and writing o*, then the batch finisher.
+// replica, reading B.nx.. and writing o*; ang_var is adapted in the launch.
 struct MCBatchSlab {
   const int4 *wg;   // per workgroup {replica, y0, z0, 0}
   const int2 *geo;  // per replica {ty, cz}

@@ -298,8 +298,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
 // Batched replicas: workgroup b sweeps tile wg[b] = {replica, y0, z0} of its
 // replica with that replica's tile shape geo[r] = {ty, cz}; the replicas'
 // planes sit at rep_off[r] in one plane set (read) and another (written).
-// Each replica evolves exactly as alone.  Counts go to the replica's
-// acc_count; the batch finisher adapts ang_var after the launch.
+// Each replica evolves exactly as alone.  The last of a replica's workgroups
+// to finish adapts its ang_var (mcd_finish_replica).
 template <int POT>
 __global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *__restrict__ params, MCDevState *states,
                                                             const int4 *__restrict__ wg, const int2 *__restrict__ geo,
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *_
   const mcsu::PassConst kc = mcsu::pass_const(P, st);
   const uint32_t acc = slab_tile<POT, false, false>(P, kc, g, in, out, d.y, d.z, MCRecord{});
   const uint32_t bsum = slab_block_sum(acc);
-  if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
+  if (threadIdx.x == 0) mcd_finish_replica(P, st, bsum, (uint32_t)((P.ny / sh.x) * (P.nz / sh.y)));
 }
 
 template <int POT>
@@ -414,5 +414,4 @@ void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *o
       break;
   }
 #undef MC_BSLAB
-  mc_launch_batch_finish(B);
 }

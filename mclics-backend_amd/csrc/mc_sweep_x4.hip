@@ -394,16 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_finish(const MCDevParams *__re
   for (int r = threadIdx.x; r < nrep; r += kBlock) {
     MCDevState *st = states + r;
     const uint32_t acc = st->acc_count;
-    float a = st->ang_var;
-    const double rate = 1.0 * (double)acc / (double)params[r].valid;
-    if (rate < 0.5) {
-      a = (float)((double)a * 0.99);
-      if ((double)a < 0.01) a = (float)0.01;
-    } else if (rate > 0.5) {
-      a = (float)((double)a / 0.99);
-      if ((double)a > 1.0) a = (float)((double)a - 0.5);
-    }
-    st->ang_var = a;
+    st->ang_var = mcd_adapt_ang_var(st->ang_var, acc, params[r].valid);
     st->last_acc = acc;
     st->acc_count = 0u;
     st->sweep = st->sweep + 1u;
