@@ -61,7 +61,7 @@ def cpu_baseline(n0, budget_s=15.0):
             "value_1core": one}
 
 
-def learned_leg(n_meas=30000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14)):
+def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14)):
     """Config C5: 128^3 LL near the transition.  The network is trained on
     n_train neighbourhoods drawn from plain-sampler lattices at the three
     temperatures, then plain and learned chains start from the same
@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
     ap.add_argument("--no-slab-geometry", action="store_true", help="skip the GHRL slab-geometry leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 30^3 thermal run")
-    ap.add_argument("--learned-sweeps", type=int, default=30000,
+    ap.add_argument("--learned-sweeps", type=int, default=100000,
                     help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
                     help="sweep kernel of the headline leg (auto: slab where it fits)")
