@@ -92,3 +92,16 @@ def test_create_validates_before_device(bad):
     with pytest.raises(mclics.EngineError) as ei:
         mclics.Lattice(cfg, n, pt)
     assert ei.value.code == -1
+
+
+@pytest.mark.parametrize("xi,xf,dx", [(1.2, 0.0, -0.02), (0.0, 1.0, 0.05), (1.2, 1.0, -0.1), (0.5, 0.5, 0.1),
+                                      (2.0, 0.1, -0.3)])
+def test_bench_float_loop_matches_the_oracle(xi, xf, dx):
+    # bench.py's temperature list for the 30^3 thermal run follows the
+    # reference's FP32 float loop; the oracle's C restatement is the check.
+    import _oracle as O
+    import bench
+    rows, ref = O.float_loop(xi, xf, dx)
+    got = bench.float_loop(xi, xf, dx)
+    assert len(got) == rows
+    assert np.array_equal(np.asarray(got, np.float32), ref)
