@@ -100,6 +100,8 @@ struct mc_handle {
   int series_cap = 0;
   float *staging = nullptr;  // AoS float[3*Nt]
   int measure_grid = 0;
+  bool measure_slab = false;  // measurement on the slab layout (shape mshape)
+  MCSlabShape mshape{0, 0, 0};
   // Parity and record buffers, allocated on first use.
   float *pbuf = nullptr;     // u / eo / en / records: 7 float planes
   uint8_t *abuf = nullptr;   // accept flags
@@ -278,7 +280,9 @@ int enqueue_sweep(mc_handle *h, const MCRecord *rec, const MCLearnRecord *lrec =
 // per-site kernel and its one-block finisher.
 void launch_measure(mc_handle *h, double *series) {
   const MCLaunch L = h->launch();
-  if (mc_x4_eligible(L))  // any sweep order: the measurement reads the lattice only
+  if (h->measure_slab)  // any sweep order: the measurement reads the lattice only
+    mc_launch_measure_slab(L, h->mshape, h->partials, h->result, series);
+  else if (mc_x4_eligible(L))
     mc_launch_measure_x4(L, h->measure_grid, h->measure_grid % 8 == 0 ? 1 : 0, h->partials, h->result, series);
   else
     mc_launch_measure(L, h->measure_grid, h->partials, h->result, series);
@@ -537,6 +541,13 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
     (void)hipFree(tmp);
     MC_HIPC(e1);
     MC_HIPC(e2);
+  }
+  // Measurement streams z-planes through LDS wherever the x4 layout applies
+  // and a tile shape fits; the segment measurement covers the rest.
+  if (mc_x4_eligible(h->launch()) &&
+      mc_measure_slab_shape(h->P, h->lds_limit, h->cu_count, h->measure_grid, &h->mshape)) {
+    h->measure_slab = true;
+    (void)mc_slab_prepare(h->lds_limit);
   }
   k_init_state<<<1, 64, 0, h->stream>>>(h->st, p->T, p->elecE, p->ang_var, h->ctr);
   MC_HIPC(hipGetLastError());

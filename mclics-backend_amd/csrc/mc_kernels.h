@@ -54,6 +54,12 @@ struct MCSlabShape {
 size_t mc_slab_lds_bytes(const MCDevParams &P, int ty);
 bool mc_slab_shape_ok(const MCDevParams &P, const MCSlabShape &s, size_t lds_limit);
 int mc_slab_prepare(size_t lds_bytes);  // number of instantiations that refused the LDS attribute
+// Measurement on the slab layout (mc_sweep_slab.hip): same eligibility as the
+// fast kernels; the shape has ty rows and cz planes per workgroup and at most
+// max_grid workgroups (the partials buffer).  out/series as mc_launch_measure.
+size_t mc_measure_slab_lds_bytes(const MCDevParams &P, int ty);
+bool mc_measure_slab_shape(const MCDevParams &P, size_t lds_limit, int cu_count, int max_grid, MCSlabShape *out);
+void mc_launch_measure_slab(const MCLaunch &L, const MCSlabShape &s, double *partials, double *out, double *series);
 void mc_launch_sweep_slab(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabShape &s,
                           const MCRecord *rec);
 // Batched replicas (mc_sweep_x4.hip / mc_batch.hip).

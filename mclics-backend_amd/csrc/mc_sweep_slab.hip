@@ -322,6 +322,133 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *_
   if (threadIdx.x == 0) mcd_finish_replica(P, st, bsum, (uint32_t)((P.ny / sh.x) * (P.nz / sh.y)));
 }
 
+// ---------------------------------------------------------------------------
+// Measurement on the slab layout: mean site energy (each site's energy with
+// the reference arithmetic, src/geometry.cpp:19-72: FP32 bonds summed in
+// FP64 and rounded, then surface and field) and the FP64 Q tensor, with the
+// lattice read once.  A workgroup owns TY full rows of CZ planes and streams
+// z: threads hold their 16-byte segment of rows y0-1 .. y0+TY (periodic) for
+// planes p-1, p, p+1 in registers (plane p+2 in flight), and plane p's rows
+// go through LDS (double buffered by plane parity, one barrier per plane) for
+// the y neighbours and the x neighbours beyond the segment.  Partial sums
+// {E, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n} per workgroup; the one-block finisher
+// reduces them in workgroup order.
+__device__ __forceinline__ void slab_block_sum8(const double a[8], double *partials) {
+  __shared__ double red[8][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double v = a[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double v = 0;
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) v += red[threadIdx.x][w];
+    partials[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+  }
+}
+
+template <int POT, bool GEO>
+__global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, const float *__restrict__ inx,
+                                                              const float *__restrict__ iny,
+                                                              const float *__restrict__ inz, const MCDevState *st,
+                                                              int TY, int CZ, double *__restrict__ partials) {
+  extern __shared__ float lds[];
+  const int X = P.nx, Y = P.ny, Z = P.nz, S = X >> 2, R = TY + 2, XY = X * Y;
+  const int nyt = Y / TY;
+  const int y0 = ((int)blockIdx.x % nyt) * TY, z0 = ((int)blockIdx.x / nyt) * CZ;
+  const int q = threadIdx.x % S, r = threadIdx.x / S;
+  const bool active = r < R, owned = r >= 1 && r <= TY;
+  const int y = ((y0 - 1 + r) % Y + Y) % Y;
+  const int rowbase = X * y + 4 * q;
+  const float *const in[3] = {inx, iny, inz};
+  const size_t H = (size_t)3 * R * X;  // floats per buffer
+  const float elecE = st->elecE;
+  const double cA = (double)(-P.A) * 1.5;
+  const bool zfree = GEO && P.zfree;
+  auto load = [&](int z, float4 v[3]) {
+    if (zfree && (z < 0 || z >= Z)) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      return;
+    }
+    const int zz = z < 0 ? z + Z : (z >= Z ? z - Z : z);
+    const int off = XY * zz + rowbase;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[d] = *reinterpret_cast<const float4 *>(in[d] + off);
+  };
+  float4 Pm[3], Pc[3], Pp[3];
+  if (active) {
+    load(z0 - 1, Pm);
+    load(z0, Pc);
+    load(z0 + 1, Pp);
+  }
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int it = 0; it < CZ; ++it) {
+    const int p = z0 + it;
+    float *B = lds + (size_t)(p & 1) * H;
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) *reinterpret_cast<float4 *>(B + d * R * X + r * X + 4 * q) = Pc[d];
+    }
+    __syncthreads();
+    float4 Pn[3];
+    if (active && it + 1 < CZ) load(p + 2, Pn);  // in flight while plane p is measured
+    if (owned) {
+      const bool zp_on = !zfree || p + 1 < Z, zm_on = !zfree || p > 0;
+      const int surf = GEO && P.zsurf ? (p == 0 ? 0 : (p == Z - 1 ? 1 : -1)) : -1;
+      float4 yp[3], ym[3];
+      float ol[3], orr[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float *row = B + d * R * X + r * X;
+        yp[d] = *reinterpret_cast<const float4 *>(row + X + 4 * q);
+        ym[d] = *reinterpret_cast<const float4 *>(row - X + 4 * q);
+        ol[d] = row[q == 0 ? X - 1 : 4 * q - 1];
+        orr[d] = row[q == S - 1 ? 0 : 4 * q + 4];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float o[3], m[6][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          o[d] = sel(Pc[d], e);
+          m[0][d] = e == 3 ? orr[d] : sel(Pc[d], e + 1);  // +x
+          m[1][d] = e == 0 ? ol[d] : sel(Pc[d], e - 1);   // -x
+          m[2][d] = sel(yp[d], e);
+          m[3][d] = sel(ym[d], e);
+          m[4][d] = sel(Pp[d], e);
+          m[5][d] = sel(Pm[d], e);
+        }
+        double so = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+          if ((b == 4 && !zp_on) || (b == 5 && !zm_on)) continue;
+          so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+        }
+        float E = (float)so;
+        if (GEO && surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
+        if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
+        const double xx = o[0], yy = o[1], zz = o[2];
+        a[0] += (double)E;
+        a[1] += 3.0 * xx * xx - 1.0;
+        a[2] += 3.0 * xx * yy;
+        a[3] += 3.0 * xx * zz;
+        a[4] += 3.0 * yy * yy - 1.0;
+        a[5] += 3.0 * yy * zz;
+        a[6] += 3.0 * zz * zz - 1.0;
+        a[7] += 1.0;
+      }
+    }
+    copy3(Pm, Pc);
+    copy3(Pc, Pp);
+    if (it + 1 < CZ) copy3(Pp, Pn);
+  }
+  slab_block_sum8(a, partials);
+}
+
 template <int POT>
 void launch_t(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabShape &s, const MCRecord *rec) {
   const int threads = (L.P.nx / 4) * (s.ty + 4);
@@ -352,6 +479,57 @@ bool mc_slab_shape_ok(const MCDevParams &P, const MCSlabShape &s, size_t lds_lim
   return mc_slab_lds_bytes(P, s.ty) + 256 <= lds_limit;  // + the static reduction array
 }
 
+size_t mc_measure_slab_lds_bytes(const MCDevParams &P, int ty) {
+  return (size_t)2 * 3 * (ty + 2) * P.nx * sizeof(float);
+}
+
+// Whole waves: the workgroup reduction shuffles across all 64 lanes, and the
+// lanes beyond the last row hold zero sums.
+static int measure_slab_threads(const MCDevParams &P, int ty) { return ((P.nx / 4) * (ty + 2) + 63) / 64 * 64; }
+
+bool mc_measure_slab_shape(const MCDevParams &P, size_t lds_limit, int cu_count, int max_grid, MCSlabShape *out) {
+  if ((P.nx % 4) != 0 || !P.per[0] || !P.per[1]) return false;
+  const int prefs[] = {8, 4, 16, 6, 2, 12, 10, 14, 1};
+  for (int ty : prefs) {
+    if (ty > P.ny || P.ny % ty) continue;
+    if (measure_slab_threads(P, ty) > kMaxThreads) continue;
+    if (mc_measure_slab_lds_bytes(P, ty) + 256 > lds_limit) continue;
+    int cz = P.nz;
+    while (cz % 2 == 0 && cz / 2 >= 4 && (long long)(P.ny / ty) * (P.nz / cz) < cu_count) cz /= 2;
+    if ((long long)(P.ny / ty) * (P.nz / cz) > max_grid) continue;
+    *out = MCSlabShape{ty, cz, 0};
+    return true;
+  }
+  return false;
+}
+
+void mc_launch_measure_slab(const MCLaunch &L, const MCSlabShape &s, double *partials, double *out, double *series) {
+  const int threads = measure_slab_threads(L.P, s.ty);
+  const int grid = (L.P.ny / s.ty) * (L.P.nz / s.cz);
+  const size_t lds = mc_measure_slab_lds_bytes(L.P, s.ty);
+#define MC_MSLAB(POT_, GEO_) \
+  k_measure_slab<POT_, GEO_><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, s.ty, s.cz, partials)
+#define MC_MSLAB_G(POT_)  \
+  if (L.P.zfree)          \
+    MC_MSLAB(POT_, true); \
+  else                    \
+    MC_MSLAB(POT_, false)
+  switch (L.P.potential) {
+    case MCD_GHRL:
+      MC_MSLAB_G(MCD_GHRL);
+      break;
+    case MCD_PEAR:
+      MC_MSLAB_G(MCD_PEAR);
+      break;
+    default:
+      MC_MSLAB_G(MCD_LL);
+      break;
+  }
+#undef MC_MSLAB_G
+#undef MC_MSLAB
+  mc_launch_measure_final(L.stream, partials, grid, out, series);
+}
+
 int mc_slab_prepare(size_t lds_bytes) {
   // Dynamic LDS beyond 64 KiB: ask for it on every instantiation.  The HIP
   // runtime may reject the attribute on devices whose per-workgroup limit is
@@ -373,6 +551,12 @@ int mc_slab_prepare(size_t lds_bytes) {
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, true>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, true>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, true>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_LL, false>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_GHRL, false>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_PEAR, false>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_LL, true>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_GHRL, true>),
+      reinterpret_cast<const void *>(&k_measure_slab<MCD_PEAR, true>),
   };
   int failed = 0;
   for (const void *f : fns)
