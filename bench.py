@@ -324,12 +324,17 @@ def main():
         if args.warmup:
             lat.sweeps(args.warmup)
         lat.sync()
-        pass_ms_sum, passes = 0.0, 0
+        # Each temperature's sweeps run back to back between one HIP event
+        # pair on the lattice's stream: the mean launch duration is the
+        # events' time over the launches (kernels plus the gaps between
+        # them).  An event pair around every launch would add its own few
+        # microseconds to each and stretch the wall clock.
+        region_ms, passes = 0.0, 0
         t0 = time.perf_counter()
         for T, k in zip(TEMPS, per_t):
             lat.set_T(T)
-            avg, _, npass = lat.time_passes(k)
-            pass_ms_sum += avg * npass
+            _, tot_ms, npass = lat.time_passes(k, 0)
+            region_ms += tot_ms
             passes += npass
         lat.sync()
         wall = time.perf_counter() - t0
@@ -337,14 +342,29 @@ def main():
             E, S, _ = lat.measure()
             obs.append({"T": T, "E": E, "S": S})
         acc, sweeps = lat.acceptance()
+        # Measured sweeps (a sweep then E and S, as in a run's MCS phase),
+        # outside the timed region: their cost beside a plain sweep.
+        n_ms = 200
+        lat.sync()
+        t1 = time.perf_counter()
+        lat.sweeps(n_ms)
+        lat.sync()
+        t2 = time.perf_counter()
+        lat.measure_series(n_ms)
+        t3 = time.perf_counter()
+        measured_step = {"sweeps": n_ms, "plain_sweep_us": (t2 - t1) / n_ms * 1e6,
+                         "measured_sweep_us": (t3 - t2) / n_ms * 1e6,
+                         "measurement_us": ((t3 - t2) - (t2 - t1)) / n_ms * 1e6,
+                         "note": "eager launches; measurement = E (FP64 sum of site energies) and the "
+                                 "FP64 Q tensor with its largest eigenvalue S"}
         kind = lat.kernel()
         kname = {mclics.KERNEL_SLAB: "k_sweep_slab (slab kernel, both colours, one launch per sweep)",
                  mclics.KERNEL_X4: "k_sweep_x4 (segment kernel, one colour pass)"}.get(
             kind, "k_sweep_pass (site kernel, one colour pass)")
         tile = lat.slab_tile() if kind == mclics.KERNEL_SLAB else None
-        log(f"bench: {args.steps} sweeps in {wall:.3f} s, mean pass {pass_ms_sum / passes * 1e3:.2f} us")
+        log(f"bench: {args.steps} sweeps in {wall:.3f} s, mean pass {region_ms / passes * 1e3:.2f} us")
     value = nt * args.steps / wall
-    avg_pass_s = pass_ms_sum / passes / 1e3
+    avg_pass_s = region_ms / passes / 1e3
     # a launch updates one colour (two-pass kernels) or the whole lattice (slab)
     launches_per_sweep = passes / args.steps
     bytes_per_launch = BYTES_PER_SITE_UPDATE * nt / launches_per_sweep
@@ -369,8 +389,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(kname.split()[0]),
                      "kernel": kname, "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_pass_s * 1e6, "launches_per_sweep": launches_per_sweep,
+                     "launches_timed": passes, "timing": "HIP event pair per temperature block "
+                                                        "of back-to-back launches",
+                     "event_ms": region_ms,
                      "slab_tile": tile},
         "observables": obs,
+        "measured_sweeps": measured_step,
         "last_acceptance": acc / nt,
     }
     if not args.no_batch:

@@ -134,10 +134,15 @@ int mc_sync(mc_handle *h);
  * elapsed milliseconds and the number of sweep-kernel launches made. */
 int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches);
 
-/* Runs n sweeps eagerly with a HIP event pair around every pass launch on
- * the handle's stream: the mean pass-kernel duration, the wall time of the
- * whole region and the number of pass launches. */
-int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, int *passes);
+/* Runs n sweeps eagerly between one HIP event pair on the handle's stream
+ * and returns the region's elapsed time and the number of pass launches.
+ * With every >= 1, every `every`-th pass launch (the first included) also
+ * gets an event pair of its own and avg_pass_ms is the mean of those.  With
+ * every = 0 no pass is bracketed, the launches run back to back as in a run,
+ * and avg_pass_ms is the region's time over its pass launches (the kernels
+ * plus the gaps between them; an event pair around a single launch adds a
+ * few microseconds of its own). */
+int mc_time_passes(mc_handle *h, int n, int every, double *avg_pass_ms, double *total_ms, int *passes);
 
 /* Mean site energy and order parameter, FP64 reductions on the device
  * (energy_calculator_GPU src/evolve.cu:390-436 plus Matrice_constructor /

@@ -906,23 +906,26 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   return MC_OK;
 }
 
-int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, int *passes) {
+int mc_time_passes(mc_handle *h, int n, int every, double *avg_pass_ms, double *total_ms, int *passes) {
   int rc = check_handle(h);
   if (rc) return rc;
-  if (n < 1 || !avg_pass_ms) return fail(MC_EINVAL, "mc_time_passes: need n >= 1 and an output");
+  if (n < 1 || every < 0 || !avg_pass_ms)
+    return fail(MC_EINVAL, "mc_time_passes: need n >= 1, every >= 0 and an output");
   int np = 0;
   const long long sweeps0 = h->host_sweeps;
   for (int w = 0; w < n; ++w, ++h->host_sweeps) np += passes_per_sweep(h);  // the loop below, dry
   h->host_sweeps = sweeps0;
-  std::vector<hipEvent_t> ev(2 * (size_t)np, nullptr);
+  const int nt = every ? (np + every - 1) / every : 0;  // bracketed passes: q = 0, every, 2 every, ...
+  std::vector<hipEvent_t> ev(2 * (size_t)nt, nullptr);
   for (auto &e : ev) MC_HIP(hipEventCreate(&e));
   MC_HIP(hipEventRecord(h->ev0, h->stream));
   for (int w = 0, q = 0; w < n; ++w) {
     const int per = passes_per_sweep(h);
     for (int t = 0; t < per; ++t, ++q) {
-      MC_HIP(hipEventRecord(ev[2 * q], h->stream));
+      const bool timed = every && q % every == 0;
+      if (timed) MC_HIP(hipEventRecord(ev[2 * (q / every)], h->stream));
       launch_pass(h, t, nullptr);
-      MC_HIP(hipEventRecord(ev[2 * q + 1], h->stream));
+      if (timed) MC_HIP(hipEventRecord(ev[2 * (q / every) + 1], h->stream));
     }
     ++h->host_sweeps;
   }
@@ -930,15 +933,15 @@ int mc_time_passes(mc_handle *h, int n, double *avg_pass_ms, double *total_ms, i
   MC_HIP(hipEventRecord(h->ev1, h->stream));
   MC_HIP(hipEventSynchronize(h->ev1));
   double sum = 0.0;
-  for (int q = 0; q < np; ++q) {
+  for (int i = 0; i < nt; ++i) {
     float ms = 0.0f;
-    MC_HIP(hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+    MC_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
     sum += ms;
   }
   float tot = 0.0f;
   MC_HIP(hipEventElapsedTime(&tot, h->ev0, h->ev1));
   for (auto &e : ev) (void)hipEventDestroy(e);
-  *avg_pass_ms = sum / np;
+  *avg_pass_ms = nt ? sum / nt : (double)tot / np;
   if (total_ms) *total_ms = tot;
   if (passes) *passes = np;
   return MC_OK;

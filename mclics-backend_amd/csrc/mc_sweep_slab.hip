@@ -431,21 +431,29 @@ __global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, con
         float E = (float)so;
         if (GEO && surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
         if (P.elecA != 0.0f) E += mcd_field(P, elecE, o[0], o[1], o[2]);
+        // second moments: the product of two FP32 values is exact in FP64
         const double xx = o[0], yy = o[1], zz = o[2];
         a[0] += (double)E;
-        a[1] += 3.0 * xx * xx - 1.0;
-        a[2] += 3.0 * xx * yy;
-        a[3] += 3.0 * xx * zz;
-        a[4] += 3.0 * yy * yy - 1.0;
-        a[5] += 3.0 * yy * zz;
-        a[6] += 3.0 * zz * zz - 1.0;
-        a[7] += 1.0;
+        a[1] += xx * xx;
+        a[2] += xx * yy;
+        a[3] += xx * zz;
+        a[4] += yy * yy;
+        a[5] += yy * zz;
+        a[6] += zz * zz;
       }
+      a[7] += 4.0;
     }
     copy3(Pm, Pc);
     copy3(Pc, Pp);
     if (it + 1 < CZ) copy3(Pp, Pn);
   }
+  // the per-site terms of Q: 3 n_a n_b - delta_ab
+  a[1] = 3.0 * a[1] - a[7];
+  a[4] = 3.0 * a[4] - a[7];
+  a[6] = 3.0 * a[6] - a[7];
+  a[2] *= 3.0;
+  a[3] *= 3.0;
+  a[5] *= 3.0;
   slab_block_sum8(a, partials);
 }
 
@@ -489,13 +497,15 @@ static int measure_slab_threads(const MCDevParams &P, int ty) { return ((P.nx / 
 
 bool mc_measure_slab_shape(const MCDevParams &P, size_t lds_limit, int cu_count, int max_grid, MCSlabShape *out) {
   if ((P.nx % 4) != 0 || !P.per[0] || !P.per[1]) return false;
-  const int prefs[] = {8, 4, 16, 6, 2, 12, 10, 14, 1};
+  // Four-row tiles, at least two per CU: on 256^3 and 128^3 LL this timed
+  // about 8 % faster than eight-row tiles with one per CU.
+  const int prefs[] = {4, 8, 16, 6, 2, 12, 10, 14, 1};
   for (int ty : prefs) {
     if (ty > P.ny || P.ny % ty) continue;
     if (measure_slab_threads(P, ty) > kMaxThreads) continue;
     if (mc_measure_slab_lds_bytes(P, ty) + 256 > lds_limit) continue;
     int cz = P.nz;
-    while (cz % 2 == 0 && cz / 2 >= 4 && (long long)(P.ny / ty) * (P.nz / cz) < cu_count) cz /= 2;
+    while (cz % 2 == 0 && cz / 2 >= 4 && (long long)(P.ny / ty) * (P.nz / cz) < 2LL * cu_count) cz /= 2;
     if ((long long)(P.ny / ty) * (P.nz / cz) > max_grid) continue;
     *out = MCSlabShape{ty, cz, 0};
     return true;

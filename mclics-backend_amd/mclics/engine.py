@@ -162,7 +162,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_sweeps_graph.argtypes = [vp, C.c_int]
     lib.mc_sync.argtypes = [vp]
     lib.mc_time_sweeps.argtypes = [vp, C.c_int, C.c_int, f32p, C.POINTER(C.c_int)]
-    lib.mc_time_passes.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.mc_time_passes.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]
     lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_measure_series.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_site_energies.argtypes = [vp, f32p]
@@ -401,9 +402,13 @@ class Lattice:
         _check(self._lib.mc_time_sweeps(self._h, n, 1 if graph else 0, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
 
-    def time_passes(self, n: int):
+    def time_passes(self, n: int, every: int = 1):
+        """n eager sweeps between one event pair.  every >= 1 also brackets
+        every `every`-th pass launch and averages those; every = 0 averages
+        the region over its pass launches.  Returns (mean pass ms, region ms,
+        pass launches)."""
         avg, tot, np_ = C.c_double(), C.c_double(), C.c_int()
-        _check(self._lib.mc_time_passes(self._h, n, C.byref(avg), C.byref(tot), C.byref(np_)))
+        _check(self._lib.mc_time_passes(self._h, n, every, C.byref(avg), C.byref(tot), C.byref(np_)))
         return avg.value, tot.value, np_.value
 
     def measure(self):
