@@ -204,6 +204,53 @@ def test_measure_matches_oracle():
         assert S == pytest.approx(np.linalg.eigvalsh(Q)[-1], abs=1e-12)
 
 
+def _measure_case(dims, pot, zfree, faces, rng, **kw):
+    nx, ny, nz = dims
+    nt = nx * ny * nz
+    surf = [("homeotropic_ghrl", 3.0, (0.0, 0.0, 1.0)), ("homeotropic_ghrl", 2.0, (0.0, 0.0, 1.0))] if faces else []
+    bound = (1, 1, 0 if zfree else 1)
+    po = O.params(nx, ny, nz, bound=bound, potential=pot, surfaces=surf, **kw)
+    pt = np.ones(nt, np.int32)
+    ns = None
+    if faces:
+        ns = np.zeros(3 * nt, np.float32)
+        pt[: nx * ny] = 2
+        pt[nx * ny * (nz - 1):] = 3
+        ns[2: 3 * nx * ny: 3] = -1.0
+        ns[3 * nx * ny * (nz - 1) + 2::3] = 1.0
+    cfg = mclics.Config(nx, ny, nz, bound=bound, potential=pot, A=po.A, B1=po.B1, B2=po.B2, C=po.C,
+                        ghrl=O.ghrl_tuple(po), elec=(po.elecX, po.elecY, po.elecZ), elecA=po.elecA, elecE=po.elecE,
+                        surfaces=[mclics.Surface(k, w, d) for k, w, d in surf], T=0.9, seed=11)
+    return po, cfg, pt if faces else None, ns, unit(rng, nt)
+
+
+@pytest.mark.parametrize("dims,pot,zfree,faces,kw", [
+    ((4, 2, 2), O.LL, False, False, {"A": 1.0}),                       # one tile, one segment per row
+    ((12, 6, 10), O.LL, False, False, {"A": 1.3}),                     # 6-row tiles, 3 segments per row
+    ((128, 12, 4), O.LL, False, False, {"A": 1.0, "elec": (0.2, 0.1, 1.0), "elecA": 0.5, "elecE": 0.8}),
+    ((64, 16, 8), O.GHRL, False, False, {"k": GHRL_K, "elec": (0.3, -0.2, 1.1), "elecA": 0.8, "elecE": 0.7}),
+    ((16, 10, 7), O.PEAR, True, False, {"A": 1.0, "B1": 0.3, "Cc": 0.3}),  # free z, odd plane count
+    ((32, 8, 6), O.GHRL, True, True, {"k": (1.48, 1.0, 2.28, 0.0), "elec": (0.0, 0.0, 1.0), "elecA": 1.0,
+                                      "elecE": 0.45}),                 # anchoring faces
+])
+def test_measure_shapes_match_oracle(dims, pot, zfree, faces, kw):
+    """E, Q and S over tile shapes of the plane-streaming measurement (row
+    tiles of 1-8, plane chunks, partial waves, free z, surface faces)."""
+    rng = np.random.default_rng(sum(dims) + pot)
+    po, cfg, pt, ns, n = _measure_case(dims, pot, zfree, faces, rng, **kw)
+    ptn = pt if pt is not None else np.ones(len(n) // 3, np.int32)
+    with mclics.Lattice(cfg, n, pt, ns) as lat:
+        for _ in range(2):
+            lat.sweeps(2)
+            d = lat.directors()
+            E, S, q = lat.measure()
+            assert E == pytest.approx(O.energy_mean(po, d, ptn, ns, gpu=1), rel=1e-12, abs=1e-12)
+            qo = O.qtensor(d, ptn)
+            np.testing.assert_allclose(q, qo, rtol=1e-11, atol=1e-13)
+            Q = np.array([[qo[0], qo[1], qo[2]], [qo[1], qo[3], qo[4]], [qo[2], qo[4], qo[5]]])
+            assert S == pytest.approx(np.linalg.eigvalsh(Q)[-1], abs=1e-12)
+
+
 def test_graph_replay_matches_eager():
     rng = np.random.default_rng(14)
     nx, ny, nz = 16, 12, 10
