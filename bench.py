@@ -139,10 +139,17 @@ def batch_leg(n_sweeps=400):
         ms = b.time_sweeps(n_sweeps)
         kernel = {mclics.KERNEL_SLAB: "k_batch_slab", mclics.KERNEL_X4: "k_batch_x4"}.get(b.kernel(), "?")
         E, S = b.measure()
+        n_ms = 50  # measured sweeps: a sweep, then E and S of every replica
+        b.measure_series(2)  # allocates the series buffer
+        b.sync()
+        t0 = time.perf_counter()
+        b.measure_series(n_ms)
+        measured_us = (time.perf_counter() - t0) / n_ms * 1e6
     rate = n_sweeps * tot / (ms / 1e3)
     return {"workload": "64 LL replicas periodic: 16 each of 32^3, 48^3, 64^3, 96^3; T_r ~ U[0.9, 1.3]",
             "sites": tot, "sweeps": n_sweeps, "us_per_sweep": ms * 1e3 / n_sweeps, "kernel": kernel,
             "site_updates_per_s": rate, "roofline_frac_24B": rate * BYTES_PER_SITE_UPDATE / (HBM_PEAK_GBS * 1e9),
+            "measured_sweep_us": measured_us, "measured_sweep_note": "eager sweep then E, S of every replica",
             "S_mean": float(np.mean(S)), "E_mean": float(np.mean(E))}
 
 

@@ -83,6 +83,13 @@ struct mc_batch {
   MCBatchSlab S{};
   int slab_budget = 0, slab_cz = 0;
   size_t lds_limit = 65536;
+  // Measurement on the slab tiles: each replica's first workgroup and
+  // workgroup count in the table, the launch shape, and the capacity of
+  // d_partials in workgroups (or segment tiles).
+  int *d_swg0 = nullptr, *d_snwg = nullptr;
+  int mthreads = 0;
+  size_t mlds = 0;
+  int partials_cap = 0;
 
   MCBatchLaunch launch() const {
     MCBatchLaunch B;
@@ -113,7 +120,7 @@ void free_batch(mc_batch *b) {
   for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   void *ptrs[] = {b->d_params, b->d_states, b->d_tile_rep, b->d_rep_tile0, b->d_rep_ntiles, b->d_rep_off, b->nx,
                   b->ny, b->nz, b->d_done, b->d_partials, b->d_out, b->d_series, b->staging, b->bx, b->by, b->bz,
-                  b->d_wg, b->d_geo};
+                  b->d_wg, b->d_geo, b->d_swg0, b->d_snwg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -190,6 +197,24 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target) {
   if (!b->d_geo) MB_HIP(hipMalloc(&b->d_geo, b->nrep * sizeof(int2)));
   MB_HIP(hipMemcpy(b->d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice));
   MB_HIP(hipMemcpy(b->d_geo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice));
+  // the table lists each replica's workgroups consecutively
+  std::vector<int> wg0(b->nrep, 0), nwg(b->nrep, 0);
+  for (int w = (int)wg.size() - 1; w >= 0; --w) {
+    wg0[wg[w].x] = w;
+    ++nwg[wg[w].x];
+  }
+  if (!b->d_swg0) MB_HIP(hipMalloc(&b->d_swg0, b->nrep * sizeof(int)));
+  if (!b->d_snwg) MB_HIP(hipMalloc(&b->d_snwg, b->nrep * sizeof(int)));
+  MB_HIP(hipMemcpy(b->d_swg0, wg0.data(), b->nrep * sizeof(int), hipMemcpyHostToDevice));
+  MB_HIP(hipMemcpy(b->d_snwg, nwg.data(), b->nrep * sizeof(int), hipMemcpyHostToDevice));
+  if ((int)wg.size() > b->partials_cap) {
+    double *np = nullptr;
+    MB_HIP(hipMalloc(&np, wg.size() * 8 * sizeof(double)));
+    if (b->d_partials) MB_HIP(hipFree(b->d_partials));
+    b->d_partials = np;
+    b->partials_cap = (int)wg.size();
+  }
+  b->mlds = mc_batch_measure_slab_shape(b->hp.data(), geo.data(), b->nrep, &b->mthreads);
   b->S = MCBatchSlab{b->d_wg, b->d_geo, (int)wg.size(), threads, lds};
   b->slab_budget = budget;
   b->slab_cz = cz_target;
@@ -368,6 +393,7 @@ int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_ao
   MB_HIPC(hipMalloc(&b->d_rep_off, nrep * sizeof(long long)));
   MB_HIPC(hipMalloc(&b->d_done, sizeof(uint32_t)));
   MB_HIPC(hipMalloc(&b->d_partials, (size_t)b->ntiles * 8 * sizeof(double)));
+  b->partials_cap = b->ntiles;
   MB_HIPC(hipMalloc(&b->d_out, (size_t)nrep * 8 * sizeof(double)));
   MB_HIPC(hipMemcpy(b->d_params, b->hp.data(), nrep * sizeof(MCDevParams), hipMemcpyHostToDevice));
   MB_HIPC(hipMemcpy(b->d_tile_rep, tile_rep.data(), b->ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -459,10 +485,20 @@ int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
   return MC_OK;
 }
 
+// E, Q and S of every replica into d_out (and series when given): on the
+// slab tiles when the slab kernel is active, else per segment tile.
+static void launch_batch_measure(mc_batch *b, double *series) {
+  if (b->slab)
+    mc_launch_batch_measure_slab(b->launch(), b->S, b->mthreads, b->mlds, b->d_swg0, b->d_snwg, b->d_partials,
+                                 b->d_out, series);
+  else
+    mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out, series);
+}
+
 int mc_batch_measure(mc_batch *b, double *E, double *S) {
   int rc = check_batch(b);
   if (rc) return rc;
-  mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out, nullptr);
+  launch_batch_measure(b, nullptr);
   MB_HIP(hipGetLastError());
   std::vector<double> o((size_t)b->nrep * 8);
   MB_HIP(hipMemcpyAsync(o.data(), b->d_out, o.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -489,8 +525,7 @@ int mc_batch_measure_series(mc_batch *b, int n, double *E, double *S) {
   for (int w = 0; w < n; ++w) {
     enqueue_batch_sweep(b);
     // (after the sweep: the slab kernel moves the lattice to the other plane set)
-    mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out,
-                            b->d_series + 2 * (size_t)w * b->nrep);
+    launch_batch_measure(b, b->d_series + 2 * (size_t)w * b->nrep);
   }
   MB_HIP(hipGetLastError());
   std::vector<double> o(need);

@@ -94,6 +94,18 @@ void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *o
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);
 // The per-tile partial sums alone (mc_sweep_x4.hip).
 void mc_launch_batch_measure_tiles(const MCBatchLaunch &B, int grid, double *tile_partials);
+// Per-replica reduction of partial sums: replica r owns partials first[r] ..
+// first[r] + count[r] - 1, summed in that order.
+void mc_launch_batch_measure_final(const MCBatchLaunch &B, const int *first, const int *count, const double *partials,
+                                   double *out, double *series);
+// Batched measurement on the slab tiles (mc_sweep_slab.hip): the launch shape
+// for the replicas' tile shapes (returns the LDS bytes, sets *threads), then
+// the launch: one workgroup per slab tile, partials[nwg * 8], and the
+// per-replica finisher over each replica's consecutive workgroups wg0[r] ..
+// wg0[r] + nwg[r] - 1.
+size_t mc_batch_measure_slab_shape(const MCDevParams *hp, const int2 *geo, int nrep, int *threads);
+void mc_launch_batch_measure_slab(const MCBatchLaunch &B, const MCBatchSlab &S, int threads, size_t lds,
+                                  const int *wg0, const int *nwg, double *partials, double *out, double *series);
 void mc_launch_site_energy(const MCLaunch &L, int grid, float *out);
 // Local order parameter per site (NaN where pt == 0).
 void mc_launch_local_S(const MCLaunch &L, int grid, float *out);

@@ -483,5 +483,10 @@ void mc_launch_soa_to_aos(hipStream_t s, const float *x, const float *y, const f
 
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series) {
   mc_launch_batch_measure_tiles(B, grid, tile_partials);
-  k_batch_measure_final<<<B.nrep, kBlock, 0, B.stream>>>(B.rep_tile0, B.rep_ntiles, tile_partials, out, series);
+  mc_launch_batch_measure_final(B, B.rep_tile0, B.rep_ntiles, tile_partials, out, series);
+}
+
+void mc_launch_batch_measure_final(const MCBatchLaunch &B, const int *first, const int *count, const double *partials,
+                                   double *out, double *series) {
+  k_batch_measure_final<<<B.nrep, kBlock, 0, B.stream>>>(first, count, partials, out, series);
 }

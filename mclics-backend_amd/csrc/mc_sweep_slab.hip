@@ -350,22 +350,18 @@ __device__ __forceinline__ void slab_block_sum8(const double a[8], double *parti
   }
 }
 
+// The tile of TY rows from y0 and CZ planes from z0: this thread's sums in
+// a[8] (zero on entry).  Threads past the tile's TY + 2 rows only join the
+// barriers.
 template <int POT, bool GEO>
-__global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, const float *__restrict__ inx,
-                                                              const float *__restrict__ iny,
-                                                              const float *__restrict__ inz, const MCDevState *st,
-                                                              int TY, int CZ, double *__restrict__ partials) {
-  extern __shared__ float lds[];
+__device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const float *const in[3], float elecE,
+                                                  int TY, int CZ, int y0, int z0, float *lds, double a[8]) {
   const int X = P.nx, Y = P.ny, Z = P.nz, S = X >> 2, R = TY + 2, XY = X * Y;
-  const int nyt = Y / TY;
-  const int y0 = ((int)blockIdx.x % nyt) * TY, z0 = ((int)blockIdx.x / nyt) * CZ;
   const int q = threadIdx.x % S, r = threadIdx.x / S;
   const bool active = r < R, owned = r >= 1 && r <= TY;
   const int y = ((y0 - 1 + r) % Y + Y) % Y;
   const int rowbase = X * y + 4 * q;
-  const float *const in[3] = {inx, iny, inz};
   const size_t H = (size_t)3 * R * X;  // floats per buffer
-  const float elecE = st->elecE;
   const double cA = (double)(-P.A) * 1.5;
   const bool zfree = GEO && P.zfree;
   auto load = [&](int z, float4 v[3]) {
@@ -385,7 +381,6 @@ __global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, con
     load(z0, Pc);
     load(z0 + 1, Pp);
   }
-  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int it = 0; it < CZ; ++it) {
     const int p = z0 + it;
     float *B = lds + (size_t)(p & 1) * H;
@@ -454,6 +449,45 @@ __global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, con
   a[2] *= 3.0;
   a[3] *= 3.0;
   a[5] *= 3.0;
+}
+
+template <int POT, bool GEO>
+__global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, const float *__restrict__ inx,
+                                                              const float *__restrict__ iny,
+                                                              const float *__restrict__ inz, const MCDevState *st,
+                                                              int TY, int CZ, double *__restrict__ partials) {
+  extern __shared__ float lds[];
+  const int nyt = P.ny / TY;
+  const int y0 = ((int)blockIdx.x % nyt) * TY, z0 = ((int)blockIdx.x / nyt) * CZ;
+  const float *const in[3] = {inx, iny, inz};
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  measure_slab_tile<POT, GEO>(P, in, st->elecE, TY, CZ, y0, z0, lds, a);
+  slab_block_sum8(a, partials);
+}
+
+// Batched replicas (all periodic): workgroup b measures the tile wg[b] =
+// {replica, y0, z0} with the replica's sweep tile shape geo[r] = {ty, cz};
+// its partial sums go to partials[b], and the replica's workgroups are
+// consecutive, so the per-replica finisher sums them in workgroup order.
+template <int POT>
+__global__ __launch_bounds__(kMaxThreads) void k_batch_measure_slab(const MCDevParams *__restrict__ params,
+                                                                    const MCDevState *__restrict__ states,
+                                                                    const int4 *__restrict__ wg,
+                                                                    const int2 *__restrict__ geo,
+                                                                    const long long *__restrict__ rep_off,
+                                                                    const float *__restrict__ inx,
+                                                                    const float *__restrict__ iny,
+                                                                    const float *__restrict__ inz,
+                                                                    double *__restrict__ partials) {
+  extern __shared__ float lds[];
+  const int4 d = wg[blockIdx.x];
+  const int r = d.x;
+  const MCDevParams P = params[r];
+  const int2 sh = geo[r];
+  const long long off = rep_off[r];
+  const float *const in[3] = {inx + off, iny + off, inz + off};
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  measure_slab_tile<POT, false>(P, in, states[r].elecE, sh.x, sh.y, d.y, d.z, lds, a);
   slab_block_sum8(a, partials);
 }
 
@@ -540,6 +574,37 @@ void mc_launch_measure_slab(const MCLaunch &L, const MCSlabShape &s, double *par
   mc_launch_measure_final(L.stream, partials, grid, out, series);
 }
 
+size_t mc_batch_measure_slab_shape(const MCDevParams *hp, const int2 *geo, int nrep, int *threads) {
+  size_t lds = 0;
+  int t = 0;
+  for (int r = 0; r < nrep; ++r) {
+    t = std::max(t, measure_slab_threads(hp[r], geo[r].x));
+    lds = std::max(lds, mc_measure_slab_lds_bytes(hp[r], geo[r].x));
+  }
+  *threads = t;
+  return lds;
+}
+
+void mc_launch_batch_measure_slab(const MCBatchLaunch &B, const MCBatchSlab &S, int threads, size_t lds,
+                                  const int *wg0, const int *nwg, double *partials, double *out, double *series) {
+#define MC_BMSLAB(POT_)                                                                                           \
+  k_batch_measure_slab<POT_><<<S.nwg, threads, lds, B.stream>>>(B.params, B.states, S.wg, S.geo, B.rep_off, B.nx, \
+                                                                B.ny, B.nz, partials)
+  switch (B.potential) {
+    case MCD_GHRL:
+      MC_BMSLAB(MCD_GHRL);
+      break;
+    case MCD_PEAR:
+      MC_BMSLAB(MCD_PEAR);
+      break;
+    default:
+      MC_BMSLAB(MCD_LL);
+      break;
+  }
+#undef MC_BMSLAB
+  mc_launch_batch_measure_final(B, wg0, nwg, partials, out, series);
+}
+
 int mc_slab_prepare(size_t lds_bytes) {
   // Dynamic LDS beyond 64 KiB: ask for it on every instantiation.  The HIP
   // runtime may reject the attribute on devices whose per-workgroup limit is
@@ -567,6 +632,9 @@ int mc_slab_prepare(size_t lds_bytes) {
       reinterpret_cast<const void *>(&k_measure_slab<MCD_LL, true>),
       reinterpret_cast<const void *>(&k_measure_slab<MCD_GHRL, true>),
       reinterpret_cast<const void *>(&k_measure_slab<MCD_PEAR, true>),
+      reinterpret_cast<const void *>(&k_batch_measure_slab<MCD_LL>),
+      reinterpret_cast<const void *>(&k_batch_measure_slab<MCD_GHRL>),
+      reinterpret_cast<const void *>(&k_batch_measure_slab<MCD_PEAR>),
   };
   int failed = 0;
   for (const void *f : fns)
