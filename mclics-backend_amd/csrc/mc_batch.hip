@@ -82,6 +82,7 @@ struct mc_batch {
   int2 *d_geo = nullptr;
   MCBatchSlab S{};
   int slab_budget = 0, slab_cz = 0;
+  bool slab_aligned = false;
   size_t lds_limit = 65536;
   // Measurement on the slab tiles: each replica's first workgroup and
   // workgroup count in the table, the launch shape, and the capacity of
@@ -151,8 +152,8 @@ void drop_graphs(mc_batch *b) {
 // target: ty the largest divisor of Ny with (Nx/4)(ty+4) <= budget, cz the
 // largest divisor of Nz <= cz_target, both checked as for one lattice.
 // Fills the host tables; false when some replica has no such shape.
-bool slab_plan(const mc_batch *b, int budget, int cz_target, std::vector<int4> &wg, std::vector<int2> &geo,
-               int *threads, size_t *lds) {
+bool slab_plan(const mc_batch *b, int budget, int cz_target, bool aligned, std::vector<int4> &wg,
+               std::vector<int2> &geo, int *threads, size_t *lds) {
   wg.clear();
   geo.assign(b->nrep, int2{0, 0});
   *threads = 0;
@@ -161,11 +162,15 @@ bool slab_plan(const mc_batch *b, int budget, int cz_target, std::vector<int4> &
     const MCDevParams &P = b->hp[r];
     const int S = P.nx / 4;
     int ty = 0, cz = 0;
-    for (int t = P.ny; t >= 1; --t)
-      if (P.ny % t == 0 && S * (t + 4) <= budget) {
-        ty = t;
-        break;
-      }
+    // aligned: prefer a tile whose owned rows of each index parity fill
+    // whole waves (S * ceil(ty / 2) a multiple of 64), so no wave of the
+    // tile mixes the two colour layouts of a segment
+    for (int pass = aligned ? 0 : 1; pass < 2 && ty == 0; ++pass)
+      for (int t = P.ny; t >= 1; --t)
+        if (P.ny % t == 0 && S * (t + 4) <= budget && (pass == 1 || (S * ((t + 1) / 2)) % 64 == 0)) {
+          ty = t;
+          break;
+        }
     for (int c = std::min(cz_target, P.nz); c >= 1; --c)
       if (P.nz % c == 0) {
         cz = c;
@@ -183,13 +188,13 @@ bool slab_plan(const mc_batch *b, int budget, int cz_target, std::vector<int4> &
   return true;
 }
 
-// Makes (budget, cz_target) the slab shape: uploads the tables.
-int set_slab_shape(mc_batch *b, int budget, int cz_target) {
+// Makes (budget, cz_target, aligned) the slab shape: uploads the tables.
+int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
   std::vector<int4> wg;
   std::vector<int2> geo;
   int threads = 0;
   size_t lds = 0;
-  if (!slab_plan(b, budget, cz_target, wg, geo, &threads, &lds))
+  if (!slab_plan(b, budget, cz_target, aligned, wg, geo, &threads, &lds))
     return bfail(MC_EINVAL, "batched slab: no tile shape fits every replica");
   if (b->d_wg) MB_HIP(hipFree(b->d_wg));
   b->d_wg = nullptr;
@@ -218,6 +223,7 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target) {
   b->S = MCBatchSlab{b->d_wg, b->d_geo, (int)wg.size(), threads, lds};
   b->slab_budget = budget;
   b->slab_cz = cz_target;
+  b->slab_aligned = aligned;
   (void)mc_slab_prepare(lds);
   return MC_OK;
 }
@@ -233,8 +239,8 @@ int select_kernel(mc_batch *b, int kind) {
     std::vector<int2> geo;
     int threads = 0;
     size_t lds = 0;
-    if (slab_plan(b, 256, 16, wg, geo, &threads, &lds)) {
-      const int rc = set_slab_shape(b, 256, 16);
+    if (slab_plan(b, 256, 16, false, wg, geo, &threads, &lds)) {
+      const int rc = set_slab_shape(b, 256, 16, false);
       if (rc) return rc;
     } else {
       want = false;
@@ -645,6 +651,7 @@ int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us) {
     return ms * 1e3f / reps;
   };
   int best_g = 0, best_x = 0, best_budget = 0, best_cz = 0;
+  bool best_al = false;
   float best = 1e30f;
   const int mults[] = {2, 4, 8, 16, 32};
   for (int xm = 0; xm < 2 && err == hipSuccess; ++xm)
@@ -672,49 +679,51 @@ int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us) {
     const int budgets[] = {128, 192, 256, 384, 512, 768};
     const int czs[] = {8, 16, 32};
     for (int budget : budgets)
-      for (int czt : czs) {
-        if (err != hipSuccess) break;
-        std::vector<int4> wg;
-        std::vector<int2> geo;
-        int threads = 0;
-        size_t lds = 0;
-        if (!slab_plan(b, budget, czt, wg, geo, &threads, &lds)) continue;
-        if (twg) (void)hipFree(twg);
-        twg = nullptr;
-        err = hipMalloc(&twg, wg.size() * sizeof(int4));
-        if (err == hipSuccess)
-          err = hipMemcpy(twg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice);
-        if (err == hipSuccess) err = hipMemcpy(tgeo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice);
-        if (err != hipSuccess) break;
-        (void)mc_slab_prepare(lds);
-        const MCBatchSlab S{twg, tgeo, (int)wg.size(), threads, lds};
-        reset();
-        if (err != hipSuccess) break;
-        int cur = 0;
-        B.xcd = 0;
-        const float us = time_it([&]() {
-          float *io = scratch + (size_t)cur * 3 * nt, *oo = scratch + (size_t)(cur ^ 1) * 3 * nt;
-          B.nx = io;
-          B.ny = io + nt;
-          B.nz = io + 2 * nt;
-          mc_launch_batch_slab(B, S, oo, oo + nt, oo + 2 * nt);
-          cur ^= 1;
-        });
-        if (err != hipSuccess) break;
-        if (us < best) {
-          best = us;
-          best_budget = budget;
-          best_cz = czt;
-          best_g = (int)wg.size();
-          best_x = 0;
+      for (int czt : czs)
+        for (int al = 0; al < 2; ++al) {
+          if (err != hipSuccess) break;
+          std::vector<int4> wg;
+          std::vector<int2> geo;
+          int threads = 0;
+          size_t lds = 0;
+          if (!slab_plan(b, budget, czt, al != 0, wg, geo, &threads, &lds)) continue;
+          if (twg) (void)hipFree(twg);
+          twg = nullptr;
+          err = hipMalloc(&twg, wg.size() * sizeof(int4));
+          if (err == hipSuccess)
+            err = hipMemcpy(twg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice);
+          if (err == hipSuccess) err = hipMemcpy(tgeo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice);
+          if (err != hipSuccess) break;
+          (void)mc_slab_prepare(lds);
+          const MCBatchSlab S{twg, tgeo, (int)wg.size(), threads, lds};
+          reset();
+          if (err != hipSuccess) break;
+          int cur = 0;
+          B.xcd = 0;
+          const float us = time_it([&]() {
+            float *io = scratch + (size_t)cur * 3 * nt, *oo = scratch + (size_t)(cur ^ 1) * 3 * nt;
+            B.nx = io;
+            B.ny = io + nt;
+            B.nz = io + 2 * nt;
+            mc_launch_batch_slab(B, S, oo, oo + nt, oo + 2 * nt);
+            cur ^= 1;
+          });
+          if (err != hipSuccess) break;
+          if (us < best) {
+            best = us;
+            best_budget = budget;
+            best_cz = czt;
+            best_al = al != 0;
+            best_g = (int)wg.size();
+            best_x = 0;
+          }
         }
-      }
   }
   release();
   MB_HIP(err);
   if (best_budget > 0) {
     MB_HIP(hipStreamSynchronize(b->stream));
-    rc = set_slab_shape(b, best_budget, best_cz);
+    rc = set_slab_shape(b, best_budget, best_cz, best_al);
     if (rc) return rc;
     b->slab = true;
     drop_graphs(b);

@@ -203,7 +203,16 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
                                               const MCRecord &rec) {
   extern __shared__ float lds[];
   const int CZ = g.CZ;
-  const int q = threadIdx.x % g.S, r = threadIdx.x / g.S;
+  // Rows in thread order: the owned rows of even index, those of odd index,
+  // the two red halo rows, then the two rows that are only loaded.  A row's
+  // index parity fixes which two sites of each segment take a colour, so on
+  // short rows (S < 64, several rows to a wave) a wave then updates one
+  // parity with one code path, and whole waves skip the updates their rows
+  // do not need.  With S = 64 every wave is one row either way.
+  const int q = threadIdx.x % g.S, i = threadIdx.x / g.S, TY = g.R - 4, ne = (TY + 1) >> 1;
+  const int r = i < ne ? 2 + 2 * i
+                       : (i < TY ? 3 + 2 * (i - ne)
+                                 : (i == TY ? 1 : (i == TY + 1 ? g.R - 2 : (i == TY + 2 ? 0 : (i == TY + 3 ? g.R - 1 : i)))));
   const bool active = r < g.R;  // threads beyond S*R only join barriers
   const int y = ((y0 - 2 + r) % g.Y + g.Y) % g.Y;
   const bool red_row = r >= 1 && r <= g.R - 2;
@@ -357,7 +366,9 @@ template <int POT, bool GEO>
 __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const float *const in[3], float elecE,
                                                   int TY, int CZ, int y0, int z0, float *lds, double a[8]) {
   const int X = P.nx, Y = P.ny, Z = P.nz, S = X >> 2, R = TY + 2, XY = X * Y;
-  const int q = threadIdx.x % S, r = threadIdx.x / S;
+  // the TY owned rows first, then the two halo rows (see slab_tile)
+  const int q = threadIdx.x % S, i = threadIdx.x / S;
+  const int r = i < TY ? i + 1 : (i == TY ? 0 : (i == TY + 1 ? R - 1 : i));
   const bool active = r < R, owned = r >= 1 && r <= TY;
   const int y = ((y0 - 1 + r) % Y + Y) % Y;
   const int rowbase = X * y + 4 * q;
