@@ -143,7 +143,7 @@ def batch_leg(n_sweeps=400):
         kernel = {mclics.KERNEL_SLAB: "k_batch_slab", mclics.KERNEL_X4: "k_batch_x4"}.get(b.kernel(), "?")
         E, S = b.measure()
         n_ms = 50  # measured sweeps: a sweep, then E and S of every replica
-        b.measure_series(2)  # allocates the series buffer
+        b.measure_series(n_ms)  # allocates the series buffer
         b.sync()
         t0 = time.perf_counter()
         b.measure_series(n_ms)

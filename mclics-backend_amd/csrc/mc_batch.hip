@@ -84,13 +84,16 @@ struct mc_batch {
   int slab_budget = 0, slab_cz = 0;
   bool slab_aligned = false;
   size_t lds_limit = 65536;
-  // Measurement on the slab tiles: each replica's first workgroup and
-  // workgroup count in the table, the launch shape, and the capacity of
-  // d_partials in workgroups (or segment tiles).
-  int *d_swg0 = nullptr, *d_snwg = nullptr;
+  // Measurement tiles (plane streaming, independent of the sweep kernel and
+  // its shape): the workgroup table and per-replica shapes, each replica's
+  // first workgroup and workgroup count, and the launch shape.  Empty when
+  // some replica has no measurement tile (then segment tiles are used).
+  int4 *d_mwg = nullptr;
+  int2 *d_mgeo = nullptr;
+  int *d_mwg0 = nullptr, *d_mnwg = nullptr;
+  MCBatchSlab M{};
   int mthreads = 0;
   size_t mlds = 0;
-  int partials_cap = 0;
 
   MCBatchLaunch launch() const {
     MCBatchLaunch B;
@@ -121,7 +124,7 @@ void free_batch(mc_batch *b) {
   for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   void *ptrs[] = {b->d_params, b->d_states, b->d_tile_rep, b->d_rep_tile0, b->d_rep_ntiles, b->d_rep_off, b->nx,
                   b->ny, b->nz, b->d_done, b->d_partials, b->d_out, b->d_series, b->staging, b->bx, b->by, b->bz,
-                  b->d_wg, b->d_geo, b->d_swg0, b->d_snwg};
+                  b->d_wg, b->d_geo, b->d_mwg, b->d_mgeo, b->d_mwg0, b->d_mnwg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -188,6 +191,53 @@ bool slab_plan(const mc_batch *b, int budget, int cz_target, bool aligned, std::
   return true;
 }
 
+// Measurement tiles of every replica: ty the largest divisor of Ny whose
+// TY + 2 rows fit kMeasureBudget threads, cz the largest divisor of Nz up
+// to kMeasureCz; one table entry per tile, each replica's entries
+// consecutive.  (On the 64-replica batch, budgets 256-512 and cz 8-32 timed
+// within 3 % of each other; 384 was 15 % slower.)
+constexpr int kMeasureBudget = 256;
+constexpr int kMeasureCz = 16;
+bool measure_plan(const mc_batch *b, std::vector<int4> &wg, std::vector<int2> &geo, std::vector<int> &wg0,
+                  std::vector<int> &nwg) {
+  wg.clear();
+  geo.assign(b->nrep, int2{0, 0});
+  wg0.assign(b->nrep, 0);
+  nwg.assign(b->nrep, 0);
+  for (int r = 0; r < b->nrep; ++r) {
+    const MCDevParams &P = b->hp[r];
+    int ty = 0, cz = 0;
+    for (int t = P.ny; t >= 1; --t)
+      if (P.ny % t == 0 && (P.nx / 4) * (t + 2) <= kMeasureBudget &&
+          mc_measure_slab_lds_bytes(P, t) + 256 <= b->lds_limit) {
+        ty = t;
+        break;
+      }
+    for (int c = std::min(kMeasureCz, P.nz); c >= 1; --c)
+      if (P.nz % c == 0) {
+        cz = c;
+        break;
+      }
+    if (ty < 1 || cz < 1) return false;
+    geo[r] = int2{ty, cz};
+    wg0[r] = (int)wg.size();
+    for (int z0 = 0; z0 < P.nz; z0 += cz)
+      for (int y0 = 0; y0 < P.ny; y0 += ty) wg.push_back(int4{r, y0, z0, 0});
+    nwg[r] = (int)wg.size() - wg0[r];
+  }
+  return true;
+}
+
+// E, Q and S of every replica into d_out (and series when given): on the
+// measurement tiles when they exist, else per segment tile.
+void launch_batch_measure(mc_batch *b, double *series) {
+  if (b->d_mwg)
+    mc_launch_batch_measure_slab(b->launch(), b->M, b->mthreads, b->mlds, b->d_mwg0, b->d_mnwg, b->d_partials,
+                                 b->d_out, series);
+  else
+    mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out, series);
+}
+
 // Makes (budget, cz_target, aligned) the slab shape: uploads the tables.
 int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
   std::vector<int4> wg;
@@ -202,24 +252,6 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
   if (!b->d_geo) MB_HIP(hipMalloc(&b->d_geo, b->nrep * sizeof(int2)));
   MB_HIP(hipMemcpy(b->d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice));
   MB_HIP(hipMemcpy(b->d_geo, geo.data(), b->nrep * sizeof(int2), hipMemcpyHostToDevice));
-  // the table lists each replica's workgroups consecutively
-  std::vector<int> wg0(b->nrep, 0), nwg(b->nrep, 0);
-  for (int w = (int)wg.size() - 1; w >= 0; --w) {
-    wg0[wg[w].x] = w;
-    ++nwg[wg[w].x];
-  }
-  if (!b->d_swg0) MB_HIP(hipMalloc(&b->d_swg0, b->nrep * sizeof(int)));
-  if (!b->d_snwg) MB_HIP(hipMalloc(&b->d_snwg, b->nrep * sizeof(int)));
-  MB_HIP(hipMemcpy(b->d_swg0, wg0.data(), b->nrep * sizeof(int), hipMemcpyHostToDevice));
-  MB_HIP(hipMemcpy(b->d_snwg, nwg.data(), b->nrep * sizeof(int), hipMemcpyHostToDevice));
-  if ((int)wg.size() > b->partials_cap) {
-    double *np = nullptr;
-    MB_HIP(hipMalloc(&np, wg.size() * 8 * sizeof(double)));
-    if (b->d_partials) MB_HIP(hipFree(b->d_partials));
-    b->d_partials = np;
-    b->partials_cap = (int)wg.size();
-  }
-  b->mlds = mc_batch_measure_slab_shape(b->hp.data(), geo.data(), b->nrep, &b->mthreads);
   b->S = MCBatchSlab{b->d_wg, b->d_geo, (int)wg.size(), threads, lds};
   b->slab_budget = budget;
   b->slab_cz = cz_target;
@@ -398,8 +430,26 @@ int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_ao
   MB_HIPC(hipMalloc(&b->d_rep_ntiles, nrep * sizeof(int)));
   MB_HIPC(hipMalloc(&b->d_rep_off, nrep * sizeof(long long)));
   MB_HIPC(hipMalloc(&b->d_done, sizeof(uint32_t)));
-  MB_HIPC(hipMalloc(&b->d_partials, (size_t)b->ntiles * 8 * sizeof(double)));
-  b->partials_cap = b->ntiles;
+  {
+    std::vector<int4> mwg;
+    std::vector<int2> mgeo;
+    std::vector<int> mwg0, mnwg;
+    const bool mok = measure_plan(b, mwg, mgeo, mwg0, mnwg);
+    MB_HIPC(hipMalloc(&b->d_partials, (size_t)std::max(b->ntiles, mok ? (int)mwg.size() : 0) * 8 * sizeof(double)));
+    if (mok) {
+      MB_HIPC(hipMalloc(&b->d_mwg, mwg.size() * sizeof(int4)));
+      MB_HIPC(hipMalloc(&b->d_mgeo, nrep * sizeof(int2)));
+      MB_HIPC(hipMalloc(&b->d_mwg0, nrep * sizeof(int)));
+      MB_HIPC(hipMalloc(&b->d_mnwg, nrep * sizeof(int)));
+      MB_HIPC(hipMemcpy(b->d_mwg, mwg.data(), mwg.size() * sizeof(int4), hipMemcpyHostToDevice));
+      MB_HIPC(hipMemcpy(b->d_mgeo, mgeo.data(), nrep * sizeof(int2), hipMemcpyHostToDevice));
+      MB_HIPC(hipMemcpy(b->d_mwg0, mwg0.data(), nrep * sizeof(int), hipMemcpyHostToDevice));
+      MB_HIPC(hipMemcpy(b->d_mnwg, mnwg.data(), nrep * sizeof(int), hipMemcpyHostToDevice));
+      b->mlds = mc_batch_measure_slab_shape(b->hp.data(), mgeo.data(), nrep, &b->mthreads);
+      b->M = MCBatchSlab{b->d_mwg, b->d_mgeo, (int)mwg.size(), b->mthreads, b->mlds};
+      (void)mc_slab_prepare(b->mlds);
+    }
+  }
   MB_HIPC(hipMalloc(&b->d_out, (size_t)nrep * 8 * sizeof(double)));
   MB_HIPC(hipMemcpy(b->d_params, b->hp.data(), nrep * sizeof(MCDevParams), hipMemcpyHostToDevice));
   MB_HIPC(hipMemcpy(b->d_tile_rep, tile_rep.data(), b->ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -489,16 +539,6 @@ int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
   MB_HIP(hipEventSynchronize(b->ev1));
   MB_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
   return MC_OK;
-}
-
-// E, Q and S of every replica into d_out (and series when given): on the
-// slab tiles when the slab kernel is active, else per segment tile.
-static void launch_batch_measure(mc_batch *b, double *series) {
-  if (b->slab)
-    mc_launch_batch_measure_slab(b->launch(), b->S, b->mthreads, b->mlds, b->d_swg0, b->d_snwg, b->d_partials,
-                                 b->d_out, series);
-  else
-    mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out, series);
 }
 
 int mc_batch_measure(mc_batch *b, double *E, double *S) {
