@@ -365,6 +365,12 @@ __device__ __forceinline__ void slab_block_sum8(const double a[8], double *parti
 template <int POT, bool GEO>
 __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const float *const in[3], float elecE,
                                                   int TY, int CZ, int y0, int z0, float *lds, double a[8]) {
+  // Lebwohl-Lasher bonds are symmetric bit for bit: bond(i, j) takes the FP32
+  // dot n_i.n_j, whose three products and two sums are the same in either
+  // order.  So each x bond inside a segment and each z bond is computed once
+  // and used by both of its sites (4.25 bonds per site instead of 6); the
+  // other potentials evaluate all six.
+  constexpr bool SYM = POT == MCD_LL;
   const int X = P.nx, Y = P.ny, Z = P.nz, S = X >> 2, R = TY + 2, XY = X * Y;
   // the TY owned rows first, then the two halo rows (see slab_tile)
   const int q = threadIdx.x % S, i = threadIdx.x / S;
@@ -386,11 +392,23 @@ __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const fl
 #pragma unroll
     for (int d = 0; d < 3; ++d) v[d] = *reinterpret_cast<const float4 *>(in[d] + off);
   };
+  auto bond = [&](int b, const float o[3], const float m[3]) {
+    return mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[0], m[1], m[2]);
+  };
   float4 Pm[3], Pc[3], Pp[3];
+  float bzm[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // SYM: -z bonds of the plane's sites
   if (active) {
     load(z0 - 1, Pm);
     load(z0, Pc);
     load(z0 + 1, Pp);
+  }
+  if (SYM && owned) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float o[3] = {sel(Pc[0], e), sel(Pc[1], e), sel(Pc[2], e)};
+      const float m[3] = {sel(Pm[0], e), sel(Pm[1], e), sel(Pm[2], e)};
+      bzm[e] = bond(5, o, m);
+    }
   }
   for (int it = 0; it < CZ; ++it) {
     const int p = z0 + it;
@@ -415,6 +433,28 @@ __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const fl
         ol[d] = row[q == 0 ? X - 1 : 4 * q - 1];
         orr[d] = row[q == S - 1 ? 0 : 4 * q + 4];
       }
+      // SYM: bx[e] is the bond between x neighbours e-1 and e of the segment
+      // (bx[0] with the site left of it, bx[4] with the site right of it);
+      // bzp[e] the +z bond of site e
+      float bx[5], bzp[4];
+      if (SYM) {
+#pragma unroll
+        for (int e = 0; e <= 4; ++e) {
+          float o[3], m[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            o[d] = e == 4 ? sel(Pc[d], 3) : sel(Pc[d], e);
+            m[d] = e == 0 ? ol[d] : (e == 4 ? orr[d] : sel(Pc[d], e - 1));
+          }
+          bx[e] = bond(e == 4 ? 0 : 1, o, m);  // site e's -x bond; site 3's +x bond for e = 4
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float o[3] = {sel(Pc[0], e), sel(Pc[1], e), sel(Pc[2], e)};
+          const float m[3] = {sel(Pp[0], e), sel(Pp[1], e), sel(Pp[2], e)};
+          bzp[e] = bond(4, o, m);
+        }
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         float o[3], m[6][3];
@@ -432,7 +472,13 @@ __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const fl
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
           if ((b == 4 && !zp_on) || (b == 5 && !zm_on)) continue;
-          so += (double)mcd_bond<POT>(P, cA, b, o[0], o[1], o[2], m[b][0], m[b][1], m[b][2]);
+          float v;
+          if (SYM && b == 0) v = bx[e + 1];
+          else if (SYM && b == 1) v = bx[e];
+          else if (SYM && b == 4) v = bzp[e];
+          else if (SYM && b == 5) v = bzm[e];
+          else v = bond(b, o, m[b]);
+          so += (double)v;
         }
         float E = (float)so;
         if (GEO && surf >= 0) E += mcd_surface(P, surf, o[0], o[1], o[2], 0.0f, 0.0f, surf ? 1.0f : -1.0f);
@@ -448,8 +494,12 @@ __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const fl
         a[6] += zz * zz;
       }
       a[7] += 4.0;
+      if (SYM) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bzm[e] = bzp[e];  // the next plane's -z bonds
+      }
     }
-    copy3(Pm, Pc);
+    if (!SYM) copy3(Pm, Pc);
     copy3(Pc, Pp);
     if (it + 1 < CZ) copy3(Pp, Pn);
   }
