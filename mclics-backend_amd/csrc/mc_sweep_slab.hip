@@ -365,12 +365,15 @@ __device__ __forceinline__ void slab_block_sum8(const double a[8], double *parti
 template <int POT, bool GEO>
 __device__ __forceinline__ void measure_slab_tile(const MCDevParams &P, const float *const in[3], float elecE,
                                                   int TY, int CZ, int y0, int z0, float *lds, double a[8]) {
-  // Lebwohl-Lasher bonds are symmetric bit for bit: bond(i, j) takes the FP32
-  // dot n_i.n_j, whose three products and two sums are the same in either
-  // order.  So each x bond inside a segment and each z bond is computed once
-  // and used by both of its sites (4.25 bonds per site instead of 6); the
-  // other potentials evaluate all six.
-  constexpr bool SYM = POT == MCD_LL;
+  // Bonds are symmetric bit for bit: bond(i, j, r) == bond(j, i, -r) with
+  // the operations of mcd_bond.  Each product commutes; a dot with -r is the
+  // exact negation of the dot with r; a difference b - a is the exact
+  // negation of a - b; the two-term sums of GHRL's E1 commute.  So n_i.n_j,
+  // |n.r|, the GHRL cross term (n_i x n_j).r and the pear (n_j - n_i).r come
+  // out the same from either site.  Each x bond inside a segment and each z
+  // bond is therefore computed once and used by both of its sites: 4.25
+  // bonds per site instead of 6.
+  constexpr bool SYM = true;
   const int X = P.nx, Y = P.ny, Z = P.nz, S = X >> 2, R = TY + 2, XY = X * Y;
   // the TY owned rows first, then the two halo rows (see slab_tile)
   const int q = threadIdx.x % S, i = threadIdx.x / S;
