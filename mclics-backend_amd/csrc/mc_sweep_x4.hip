@@ -39,50 +39,81 @@ __device__ __forceinline__ void x4_row(int rowi, int nseg, int Y, int Z, int &j,
   j = 2 * (e - k * hy) + ((k + odd) & 1);
 }
 
-// One thread's segment update; PAR is the x parity of the first site of the
-// pass colour in this row (sites at 4q+PAR and 4q+PAR+2).
-template <int POT, bool RECORD, int PAR, bool GEO>
-__device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::PassConst &kc, float *__restrict__ nx,
-                                            float *__restrict__ ny, float *__restrict__ nz, int q, int j, int k,
-                                            const MCRecord &rec) {
+// The loads of one segment (slot t) for a pass of the given colour: the
+// segment, the pass-colour sites of its four y/z neighbour segments and the
+// x neighbour outside it.  PAR is the x parity of the first site of the pass
+// colour in the row (sites at 4q+PAR and 4q+PAR+2).  Only sites of the colour
+// the pass does not update are read from other segments, and the segment
+// itself belongs to this thread alone, so loads issued ahead of earlier
+// updates see the pass's input.
+struct SegIn {
+  float4 c[3];
+  mcsu::PairNbrs nb;
+  int q, j, k, par;
+  bool valid;
+};
+
+template <bool GEO>
+__device__ __forceinline__ void seg_load(const MCDevParams &P, float *const pl[3], int t, int total, int colour,
+                                         SegIn &s) {
+  s.valid = t < total;
+  if (!s.valid) return;
   const int X = P.nx, Y = P.ny, Z = P.nz;
   const int nseg = X >> 2;
+  s.q = t % nseg;
+  x4_row(t / nseg, nseg, Y, Z, s.j, s.k);
+  const int q = s.q, j = s.j, k = s.k;
+  s.par = (j + k + colour) & 1;
   const int row = X * (j + Y * k);
   const int jp = j + 1 == Y ? 0 : j + 1, jm = j == 0 ? Y - 1 : j - 1;
   const int kp = k + 1 == Z ? 0 : k + 1, km = k == 0 ? Z - 1 : k - 1;
   const int x0 = 4 * q;
   const int r_jp = X * (jp + Y * k) + x0, r_jm = X * (jm + Y * k) + x0;
   const int r_kp = X * (j + Y * kp) + x0, r_km = X * (j + Y * km) + x0;
-  const int own = row + x0;
-  const int out = PAR == 0 ? row + (q == 0 ? X - 1 : x0 - 1) : row + (q == nseg - 1 ? 0 : x0 + 4);
-  // free z faces: the missing neighbour is a zero vector with its bit clear
+  const int out = s.par == 0 ? row + (q == 0 ? X - 1 : x0 - 1) : row + (q == nseg - 1 ? 0 : x0 + 4);
+  // free z faces: the missing neighbour is a zero vector
+  const bool zp_on = !GEO || !P.zfree || k + 1 < Z, zm_on = !GEO || !P.zfree || k > 0;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    s.c[d] = *reinterpret_cast<const float4 *>(pl[d] + row + x0);
+    const float4 yp = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
+    const float4 ym = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
+    const float4 zp = zp_on ? *reinterpret_cast<const float4 *>(pl[d] + r_kp) : zero4;
+    const float4 zm = zm_on ? *reinterpret_cast<const float4 *>(pl[d] + r_km) : zero4;
+    if (s.par == 0) {
+      s.nb.yp[d] = make_float2(yp.x, yp.z);
+      s.nb.ym[d] = make_float2(ym.x, ym.z);
+      s.nb.zp[d] = make_float2(zp.x, zp.z);
+      s.nb.zm[d] = make_float2(zm.x, zm.z);
+    } else {
+      s.nb.yp[d] = make_float2(yp.y, yp.w);
+      s.nb.ym[d] = make_float2(ym.y, ym.w);
+      s.nb.zp[d] = make_float2(zp.y, zp.w);
+      s.nb.zm[d] = make_float2(zm.y, zm.w);
+    }
+    s.nb.ov[d] = pl[d][out];
+  }
+}
+
+// One thread's segment update from its loads.
+template <int POT, bool RECORD, int PAR, bool GEO>
+__device__ __forceinline__ uint32_t seg_update(const MCDevParams &P, const mcsu::PassConst &kc, float *const pl[3],
+                                               const SegIn &s, const MCRecord &rec) {
+  const int X = P.nx, Y = P.ny, Z = P.nz;
+  const int k = s.k;
+  const int own = X * (s.j + Y * k) + 4 * s.q;
   mcsu::Where w = mcsu::kBulk;
   const bool zp_on = !GEO || !P.zfree || k + 1 < Z, zm_on = !GEO || !P.zfree || k > 0;
   if (!zp_on) w.on &= ~(1u << 4);
   if (!zm_on) w.on &= ~(1u << 5);
   if (GEO && P.zsurf) w.surf = k == 0 ? 0 : (k == Z - 1 ? 1 : -1);
-  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float *pl[3] = {nx, ny, nz};
-  float4 c[3];
-  mcsu::PairNbrs nb;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    c[d] = *reinterpret_cast<const float4 *>(pl[d] + own);
-    const float4 yp = *reinterpret_cast<const float4 *>(pl[d] + r_jp);
-    const float4 ym = *reinterpret_cast<const float4 *>(pl[d] + r_jm);
-    const float4 zp = zp_on ? *reinterpret_cast<const float4 *>(pl[d] + r_kp) : zero4;
-    const float4 zm = zm_on ? *reinterpret_cast<const float4 *>(pl[d] + r_km) : zero4;
-    nb.yp[d] = make_float2(sel(yp, PAR), sel(yp, PAR + 2));
-    nb.ym[d] = make_float2(sel(ym, PAR), sel(ym, PAR + 2));
-    nb.zp[d] = make_float2(sel(zp, PAR), sel(zp, PAR + 2));
-    nb.zm[d] = make_float2(sel(zm, PAR), sel(zm, PAR + 2));
-    nb.ov[d] = pl[d][out];
-  }
+  float4 c[3] = {s.c[0], s.c[1], s.c[2]};
   const int sA = own + PAR;  // linear index of the first site
   mcsu::Site S[2];
   bool a[2];
   MCU4 r;
-  mcsu::pair_update<POT, PAR, GEO>(P, kc, c, nb, (uint32_t)sA, w, S, a, r);
+  mcsu::pair_update<POT, PAR, GEO>(P, kc, c, s.nb, (uint32_t)sA, w, S, a, r);
   if (a[0] || a[1]) {
     mcsu::pair_apply<PAR>(c, S, a);
 #pragma unroll
@@ -93,25 +124,41 @@ __device__ __forceinline__ uint32_t segment(const MCDevParams &P, const mcsu::Pa
 }
 
 template <int POT, bool RECORD, bool GEO>
+__device__ __forceinline__ uint32_t seg_update_any(const MCDevParams &P, const mcsu::PassConst &kc,
+                                                   float *const pl[3], const SegIn &s, const MCRecord &rec) {
+  if (s.par == 0) return seg_update<POT, RECORD, 0, GEO>(P, kc, pl, s, rec);
+  return seg_update<POT, RECORD, 1, GEO>(P, kc, pl, s, rec);
+}
+
+// PIPE: the workgroup has several tiles; the next tile's loads are issued
+// before the current segment is updated (one more segment of registers).
+template <int POT, bool RECORD, bool GEO, bool PIPE>
 __global__ __launch_bounds__(kBlock) void k_sweep_x4(MCDevParams P, float *__restrict__ nx, float *__restrict__ ny,
                                                      float *__restrict__ nz, MCDevState *st, int colour, int finalize,
                                                      int xcd, MCRecord rec) {
   const mcsu::PassConst kc = mcsu::pass_const(P, st);
   const int nseg = P.nx >> 2;
   const int total = nseg * P.ny * P.nz;
+  float *const pl[3] = {nx, ny, nz};
   uint32_t acc = 0;
   const MCTileRange tr = mcd_tiles((total + kBlock - 1) / kBlock, xcd);
-  for (int tile = tr.first; tile < tr.end; tile += tr.step) {
-    const int t = tile * kBlock + threadIdx.x;
-    if (t >= total) continue;
-    const int q = t % nseg;
-    const int rowi = t / nseg;
-    int j, k;
-    x4_row(rowi, nseg, P.ny, P.nz, j, k);
-    if (((j + k + colour) & 1) == 0)
-      acc += segment<POT, RECORD, 0, GEO>(P, kc, nx, ny, nz, q, j, k, rec);
-    else
-      acc += segment<POT, RECORD, 1, GEO>(P, kc, nx, ny, nz, q, j, k, rec);
+  if (PIPE) {
+    SegIn cur, nxt;
+    cur.valid = false;
+    if (tr.first < tr.end) seg_load<GEO>(P, pl, tr.first * kBlock + (int)threadIdx.x, total, colour, cur);
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+      nxt.valid = false;
+      if (tile + tr.step < tr.end)
+        seg_load<GEO>(P, pl, (tile + tr.step) * kBlock + (int)threadIdx.x, total, colour, nxt);
+      if (cur.valid) acc += seg_update_any<POT, RECORD, GEO>(P, kc, pl, cur, rec);
+      cur = nxt;
+    }
+  } else {
+    for (int tile = tr.first; tile < tr.end; tile += tr.step) {
+      SegIn sg;
+      seg_load<GEO>(P, pl, tile * kBlock + (int)threadIdx.x, total, colour, sg);
+      if (sg.valid) acc += seg_update_any<POT, RECORD, GEO>(P, kc, pl, sg, rec);
+    }
   }
   const uint32_t bsum = mcd_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, finalize != 0);
@@ -261,12 +308,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_measure_x4(const MCDevParams *
 
 template <int POT, bool GEO>
 void launch_g(const MCLaunch &L, int colour, int grid, int finalize, int xcd, const MCRecord *rec) {
+  // the pipelined loop when some workgroup has two or more tiles
+  const bool pipe = grid < (mc_x4_slots(L.P) + kBlock - 1) / kBlock;
   if (rec)
-    k_sweep_x4<POT, true, GEO><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
-                                                               *rec);
+    k_sweep_x4<POT, true, GEO, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize,
+                                                                      xcd, *rec);
+  else if (pipe)
+    k_sweep_x4<POT, false, GEO, true><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize,
+                                                                      xcd, MCRecord{});
   else
-    k_sweep_x4<POT, false, GEO><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize, xcd,
-                                                                MCRecord{});
+    k_sweep_x4<POT, false, GEO, false><<<grid, kBlock, 0, L.stream>>>(L.P, L.nx, L.ny, L.nz, L.st, colour, finalize,
+                                                                       xcd, MCRecord{});
 }
 
 template <int POT>
@@ -370,16 +422,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_x4(const MCDevParams *__restri
     const int total = nseg * P.ny * P.nz;
     const int t = (tile - rep_tile0[r]) * kBlock + threadIdx.x;
     uint32_t acc = 0;
-    if (t < total) {
-      const int q = t % nseg;
-      const int rowi = t / nseg;
-      int j, k;
-      x4_row(rowi, nseg, P.ny, P.nz, j, k);
-      if (((j + k + colour) & 1) == 0)
-        acc = segment<POT, false, 0, false>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
-      else
-        acc = segment<POT, false, 1, false>(P, kc, nx + off, ny + off, nz + off, q, j, k, MCRecord{});
-    }
+    float *const pl[3] = {nx + off, ny + off, nz + off};
+    SegIn sg;
+    seg_load<false>(P, pl, t, total, colour, sg);
+    if (sg.valid) acc = seg_update_any<POT, false, false>(P, kc, pl, sg, MCRecord{});
     const uint32_t bsum = mcd_block_sum(acc);
     if (threadIdx.x == 0) atomicAdd(&st->acc_count, bsum);
     __syncthreads();  // wsum of mcd_block_sum is reused by the next tile

@@ -93,6 +93,30 @@ def test_batch_slab_and_segment_kernels_agree():
         assert np.all(s1 == 15)
 
 
+def test_batch_autotuned_shape_keeps_the_chain():
+    """Whatever shape the batch autotuner keeps (segment grid, slab budget,
+    planes per tile, parity-aligned tile heights), the replicas follow the
+    same chains as with the default slab shape."""
+    sizes = [(96, 24, 8), (64, 16, 8), (48, 12, 8), (32, 32, 4), (8, 8, 8)]
+    rng = np.random.default_rng(17)
+    cfgs, ics = [], []
+    for r, (x, y, z) in enumerate(sizes):
+        cfgs.append(mclics.Config(x, y, z, T=float(np.float32(rng.uniform(0.9, 1.3))), seed=500 + r, stream=r))
+        ics.append(mclics.random_ic(x, y, z, seed=500 + r))
+    with mclics.Batch(cfgs, ics) as b1, mclics.Batch(cfgs, ics) as b2:
+        g, x, us = b2.autotune()
+        assert g > 0 and us > 0
+        for b in (b1, b2):
+            b.sweeps(5)
+            b.sweeps(4)
+        for r in range(len(sizes)):
+            assert np.array_equal(b1.directors(r), b2.directors(r)), r
+        assert all(np.array_equal(u, v) for u, v in zip(b1.state(), b2.state()))
+        E1, S1 = b1.measure()
+        E2, S2 = b2.measure()
+        assert np.array_equal(E1, E2) and np.array_equal(S1, S2)
+
+
 @pytest.mark.parametrize("pot", [O.GHRL, O.PEAR])
 def test_batch_other_potentials_match_single_lattices(pot):
     """GHRL (chiral, with a field) and pear replicas in a batch, on the slab
