@@ -373,6 +373,31 @@ def test_segment_kernel_replay(pot, T, field, kernel):
     np.testing.assert_allclose(np.linalg.norm(rec["prop"].reshape(-1, 3), axis=1), 1.0, atol=3e-7)
 
 
+@pytest.mark.parametrize("dims,pot,zfree,faces,kw", [
+    ((32, 16, 16), O.LL, False, False, {"A": 1.0}),
+    ((32, 16, 12), O.GHRL, True, True, {"k": GHRL_K, "elec": (0.0, 0.0, 1.0), "elecA": 1.0, "elecE": 0.5}),
+    ((32, 16, 16), O.PEAR, False, False, {"A": 1.0, "B1": 0.3, "Cc": 0.3}),
+])
+def test_segment_kernel_pipelined_grid_keeps_the_chain(dims, pot, zfree, faces, kw):
+    """With fewer workgroups than tiles the segment kernel runs its
+    pipelined loop (the next tile's loads in flight during an update); the
+    chain must equal the one-tile-per-workgroup kernel's, which the replay
+    tests pin to the oracle."""
+    rng = np.random.default_rng(sum(dims) + 3 * pot)
+    po, cfg, pt, ns, n = _measure_case(dims, pot, zfree, faces, rng, **kw)
+    with mclics.Lattice(cfg, n, pt, ns) as a, mclics.Lattice(cfg, n, pt, ns) as b:
+        a.set_kernel(mclics.KERNEL_X4)
+        b.set_kernel(mclics.KERNEL_X4)
+        b.set_tuning(3, 0)  # 8 tiles over 3 workgroups: 3, 3 and 2 tiles each
+        for T in (0.9, 0.3, 2.0):
+            a.set_T(T)
+            b.set_T(T)
+            a.sweeps(3)
+            b.sweeps(3)
+            assert np.array_equal(a.directors(), b.directors()), T
+            assert a.acceptance() == b.acceptance() and a.ang_var() == b.ang_var(), T
+
+
 SLAB_GEOMETRY = ["slab_" + k for k in ("homeotropic", "rp", "fg", "strong", "homeotropic_ghrl", "rp_ghrl",
                                         "fg_ghrl", "strong_ghrl")] + ["ll_slab", "ll_freez"]
 
