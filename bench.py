@@ -61,7 +61,7 @@ def cpu_baseline(n0, budget_s=15.0):
             "value_1core": one}
 
 
-def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14)):
+def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40):
     """Config C5: 128^3 LL near the transition.  The network is trained on
     n_train neighbourhoods drawn from plain-sampler lattices at the three
     temperatures, then plain and learned chains start from the same
@@ -86,7 +86,7 @@ def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
                                 seed=1000 * ti + k)
             starts[T] = lat.directors()
             lat.close()
-        losses = net.train(epochs=10, batch=2048, lr=2e-3)
+        losses = net.train(epochs=epochs, batch=2048, lr=2e-3)
         out["train_seconds"] = time.perf_counter() - t0
         out["loss_per_epoch"] = [float(v) for v in losses]
         gains = []
@@ -300,6 +300,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replica leg")
     ap.add_argument("--no-slab-geometry", action="store_true", help="skip the GHRL slab-geometry leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 30^3 thermal run")
+    ap.add_argument("--learned-epochs", type=int, default=40, help="training epochs of the learned-proposal leg")
     ap.add_argument("--learned-sweeps", type=int, default=100000,
                     help="measured sweeps per chain in the learned-proposal leg")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
@@ -416,7 +417,7 @@ def main():
         result["slab_geometry"] = slab_geometry_leg()
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
-        result["learned_proposal"] = learned_leg(args.learned_sweeps)
+        result["learned_proposal"] = learned_leg(args.learned_sweeps, epochs=args.learned_epochs)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         result["cpu_baseline"] = cpu_baseline(n0)
