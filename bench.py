@@ -61,7 +61,7 @@ def cpu_baseline(n0, budget_s=15.0):
             "value_1core": one}
 
 
-def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40):
+def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40, schedule=None):
     """Config C5: 128^3 LL near the transition.  The network is trained on
     n_train neighbourhoods drawn from plain-sampler lattices at the three
     temperatures, then plain and learned chains start from the same
@@ -86,7 +86,9 @@ def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
                                 seed=1000 * ti + k)
             starts[T] = lat.directors()
             lat.close()
-        losses = net.train(epochs=epochs, batch=2048, lr=2e-3)
+        # (epochs, learning rate) stages; Adam's moments and step carry across stages
+        losses = np.concatenate([net.train(epochs=e, batch=2048, lr=lr)
+                                 for e, lr in (schedule or [(epochs, 2e-3)])])
         out["train_seconds"] = time.perf_counter() - t0
         out["loss_per_epoch"] = [float(v) for v in losses]
         gains = []
