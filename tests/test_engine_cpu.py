@@ -105,3 +105,16 @@ def test_bench_float_loop_matches_the_oracle(xi, xf, dx):
     got = bench.float_loop(xi, xf, dx)
     assert len(got) == rows
     assert np.array_equal(np.asarray(got, np.float32), ref)
+
+
+def test_bench_refuses_more_than_one_gpu_and_too_few_steps():
+    # one GPU only: the multi-GPU decomposition is a design, not a code path;
+    # the timed sweeps are split over the four temperatures
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for args, text in ((["--gpus", "2"], "one GPU"), (["--steps", "3"], "--steps")):
+        p = subprocess.run([sys.executable, os.path.join(root, "bench.py")] + args, capture_output=True, text=True,
+                           timeout=120)
+        assert p.returncode == 2, (args, p.returncode, p.stderr)
+        assert text in p.stderr, (args, p.stderr)
+        assert p.stdout == "", args
