@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mc_expf.h"
+
 #define MCD_MAX_SURF 8
 #define MCD_BLOCK 256  // threads per workgroup of every sweep kernel
 
@@ -312,16 +314,17 @@ __device__ __forceinline__ float mcd_bond(const MCDevParams &P, double cA, int b
   return mcd_bond_pear(P, n0, n1, n2, m0, m1, m2, r0, r1, r2);
 }
 
-// Accept rule of the reference CPU sweep, u < expf(-dE/T) (src/evolve.cpp:160).
-// A hardware exp2 decides every case whose margin exceeds its error bound
-// (|x| <= 2^11 keeps the relative error of exp2(x*log2e) under 2^-12); the
-// rare near-ties, u == 0 and infinities are settled with an FP64 exp rounded
-// to FP32, i.e. the correctly rounded expf the CPU returns.
+// Accept rule of the reference CPU sweep, u < expf(-dE/T) (src/evolve.cpp:160,
+// where exp of a float argument is glibc's expf).  A hardware exp2 decides
+// every case whose margin exceeds its error bound (|x| <= 2^11 keeps the
+// relative error of exp2(x*log2e) under 2^-12, and glibc's expf is within
+// one ulp of exp); the rare near-ties, u == 0 and infinities are settled
+// with mcx_expf, which returns glibc's expf bit for bit (mc_expf.h).
 __device__ __forceinline__ bool mcd_accept(float E_new, float E_old, float T, float u) {
   float x = -(E_new - E_old) / T;
   float pf = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   if (u != 0.0f && !(fabsf(u - pf) <= pf * 2.44140625e-4f)) return u < pf;
-  return u < (float)exp((double)x);
+  return u < mcx_expf(x);
 }
 
 // ---------------------------------------------------------------------------

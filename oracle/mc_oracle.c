@@ -248,6 +248,14 @@ void orc_site_energies(const orc_params *p, const float *n, const int32_t *pt, c
       }
 }
 
+/* exp of a float argument as the reference's accept test evaluates it
+ * (src/evolve.cpp:160: exp() of a float in C++ is std::exp(float), glibc's
+ * expf), over an array; the tests use it to find the inputs where glibc's
+ * expf differs from the correctly rounded value. */
+void orc_expf_n(const float *x, float *y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = expf(x[i]);
+}
+
 /* Accept/reject with the reference's semantics (src/evolve.cpp:154-165). */
 static inline int update_site(const orc_params *p, float *n, const int32_t *pt, const float *ns, int i, int j,
                               int k, float T, const float nn[3], float u, float *eo, float *en) {

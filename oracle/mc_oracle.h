@@ -114,6 +114,7 @@ double orc_polarization(const float *n, int64_t nt);
 int orc_float_loop(float xi, float xf, float dx, float *xs, int max_rows);
 
 /* One Philox4x32-10 block (for known-answer tests). */
+void orc_expf_n(const float *x, float *y, int64_t n);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 #ifdef __cplusplus

@@ -81,6 +81,7 @@ def lib():
         L.orc_polarization.restype = C.c_double
         L.orc_float_loop.argtypes = [C.c_float, C.c_float, C.c_float, f32p, C.c_int]
         L.orc_float_loop.restype = C.c_int
+        L.orc_expf_n.argtypes = [f32p, f32p, C.c_int64]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
@@ -189,6 +190,14 @@ def float_loop(xi, xf, dx, max_rows=100000):
     xs = np.zeros(max_rows, np.float32)
     rows = lib().orc_float_loop(xi, xf, dx, f32(xs), max_rows)
     return rows, xs[:min(rows, max_rows)]
+
+
+def expf(x):
+    """glibc's expf of every element (the reference's exp(float))."""
+    a = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(a)
+    lib().orc_expf_n(f32(a), f32(y), a.size)
+    return y
 
 
 def philox(ctr, key):

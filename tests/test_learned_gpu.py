@@ -103,11 +103,39 @@ def test_training_lowers_the_loss_and_raises_acceptance():
     assert acc1 > acc0 + 0.05, (acc0, acc1)
 
 
+U32 = 2.0 ** -24  # FP32 unit roundoff
+
+
+def mh_window(dE_T, kap, lk, co, cn, x):
+    """Bound on |x_device - x| for one learned move, where the device forms
+    x = -dE*RN(1/T) + lcosh2(kappa*co) - lcosh2(kappa*cn) in FP32 with
+    kappa = __expf(log kappa) and compares u with __expf(x), and x is the
+    FP64 value from the recorded dE, log kappa, mu and proposal (u = 2^-24):
+      -dE/T in FP32: <= 2u |dE/T|;
+      kappa: relative <= (|log kappa| + 3)u (x*log2e rounded, then v_exp_f32
+        within one ulp); co, cn: three-term FP32 dots, <= 3u each; so each
+        z = kappa*c is off by <= kappa((|log kappa| + 4)|c| + 3u) + u|z|;
+      lcosh2(z) = |z| + log(1 + e^-2|z|): e^-2|z| off by <= 3u absolute,
+        the rounding of 1 + y by 2u, the hardware log by 4u, the final
+        addition by u(|z| + 1); d lcosh2/dz = tanh z, so a z error passes
+        through with factor <= 1;
+      the difference and the sum x: u|lr| + u|x|; __expf(x) relative
+        (|x| + 3)u, which is the same in log space.
+    A factor 2 covers the second-order terms."""
+    zo, zn = kap * abs(co), kap * abs(cn)
+    zerr = kap * ((abs(lk) + 4) * (abs(co) + abs(cn)) + 6) * U32 + U32 * (zo + zn)
+    lc = (20 + zo + zn) * U32
+    return 2.0 * (2 * abs(dE_T) * U32 + zerr + lc + U32 * (zo + zn) + (2 * abs(x) + 3) * U32)
+
+
 @pytest.mark.parametrize("dims,bound", [((12, 12, 12), (1, 1, 1)), ((7, 6, 6), (0, 1, 1))])
 def test_learned_sweep_decisions_are_exact_mh(dims, bound):
-    # Replays one recorded learned sweep: the oracle recomputes, in FP64, the
-    # energy change and the proposal-density ratio from the recorded mu/kappa
-    # and must reach the same decision for every move that is not a near tie.
+    # Replays one recorded learned sweep: the oracle recomputes the energy
+    # change from the reference site energies (the recorded FP32 dE must be
+    # within the FMA form's proven distance of it) and, in FP64 from the
+    # recorded dE, mu, log kappa and proposal, the log acceptance ratio; every
+    # move outside its own FP32 error window (mh_window, a few 1e-6 at
+    # kappa ~ 1) must be decided the same way.
     # The 7 x 6 x 6 lattice with a free x axis has colour-set slots past the
     # last column.
     nx, ny, nz = dims
@@ -124,7 +152,12 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound):
         rec = lat.record_learned_sweep()
         end = lat.directors()
     state = start.copy().reshape(-1, 3)
-    near, checked = 0, 0
+    T32 = float(np.float32(T))
+    near, checked, widest = 0, 0, 0.0
+    # distance of the FMA form of dE from the reference's site energies:
+    # 300u * 1.5|A| (the bound the FP32 screen of the plain sweep uses)
+    dE_tol = 300 * U32 * 1.5 * abs(po.A)
+    import ctypes as C
     for colour in (0, 1):
         sites = [i + nx * (j + ny * k) for k in range(nz) for j in range(ny) for i in range(nx)
                  if (i + j + k) % 2 == colour]
@@ -133,27 +166,33 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound):
             i, j, k = s % nx, (s // nx) % ny, s // (nx * ny)
             q = rec["prop"][3 * s:3 * s + 3]
             assert abs(np.linalg.norm(q.astype(np.float64)) - 1) < 1e-5
-            import ctypes as C
             eo = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, None)
             en = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, O.f32(q.copy()))
-            dE = float(en) - float(eo)
-            assert rec["dE"][s] == pytest.approx(dE, abs=2e-5)
+            dE_ref = float(en) - float(eo)
+            dE = float(rec["dE"][s])
+            assert abs(dE - dE_ref) <= dE_tol, (s, dE, dE_ref)
             mu = rec["mu"][3 * s:3 * s + 3].astype(np.float64)
-            kap = math.exp(float(rec["logk"][s]))
+            lk = float(rec["logk"][s])
+            kap = math.exp(lk)
             o = flat[3 * s:3 * s + 3].astype(np.float64)
-            x = -dE / T + (O.vmf_logq(mu, kap, o) - O.vmf_logq(mu, kap, q.astype(np.float64)))
+            qd = q.astype(np.float64)
+            x = -dE / T32 + (O.vmf_logq(mu, kap, o) - O.vmf_logq(mu, kap, qd))
+            win = mh_window(dE / T32, kap, lk, float(mu @ o), float(mu @ qd), x)
+            widest = max(widest, win)
             u = float(rec["u"][s])
-            if u > 0 and abs(math.log(u) - x) < 1e-3:
+            if (u > 0 and abs(math.log(u) - x) <= win) or (u == 0 and x < -80):
                 near += 1
                 acc = bool(rec["accept"][s])
             else:
                 acc = u < math.exp(min(x, 700.0))
-                assert acc == bool(rec["accept"][s]), (colour, s, x, u)
+                assert acc == bool(rec["accept"][s]), (colour, s, x, u, win)
             checked += 1
             if acc:
                 state[s] = q
         # next colour sees this colour's accepted moves
-    assert near < 0.01 * checked
+    print(f"learned MH replay: {checked} moves, {near} inside their window, widest window {widest:.2e}")
+    assert widest < 1e-4
+    assert near <= max(1, checked // 2000)
     np.testing.assert_array_equal(state.reshape(-1), end)
 
 
