@@ -155,6 +155,26 @@ int mc_measure(mc_handle *h, double *E, double *S, double *q6);
  * the per-sweep values.  This is the MCS loop of the mode drivers
  * (src/evolve_thermal.cu:85-95) with the lattice never leaving the device. */
 int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series);
+/* Allocation-free measurement.  No counterpart in the reference: an entry
+ * point for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * measures a lattice the caller holds on the current device as three FP32
+ * SoA planes nx, ny, nz (site i + Nx*(j + Ny*k)) and writes
+ * out[9] = {E, S, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz, n_active} (device memory),
+ * asynchronously on `stream` (a hipStream_t; NULL for the null stream).  For
+ * the same lattice the values equal mc_measure's bit for bit (same kernels,
+ * same launch shape).  Lattices: every site active, Nx a multiple of 4, x
+ * and y periodic (even Ny); z periodic (even Nz, no surfaces) or free, with
+ * no surfaces or with surfaces 0 and 1 on the planes k = 0 and k = Nz-1 (the
+ * reference's slab, src/geometry_slab.cpp:47-67); the field strength is
+ * params->elecE.  It calls no hipMalloc or hipFree: its partial sums live in
+ * a fixed device array of MC_NOALLOC_MAX_PARTIALS rows of 8 doubles per GPU
+ * (256 KiB, held until the process ends); a lattice needing more rows is
+ * rejected with MC_EINVAL.  Calls that overlap on one GPU share that array
+ * and race: order them. */
+#define MC_NOALLOC_MAX_PARTIALS 4096
+int mc_measure_noalloc(const mc_params *params, const float *nx, const float *ny, const float *nz, double *out,
+                       void *stream);
 /* FP32 site energies of every site (0 at pt==0 sites), float[Nt]. */
 int mc_site_energies(mc_handle *h, float *out);
 

@@ -51,6 +51,7 @@ EXPORTS = (
     "mc_time_passes",
     "mc_measure",
     "mc_measure_series",
+    "mc_measure_noalloc",
     "mc_site_energies",
     "mc_local_order",
     "mc_get_directors_aos",
@@ -166,6 +167,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                    C.POINTER(C.c_int)]
     lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_measure_series.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mc_measure_noalloc.argtypes = [C.POINTER(McParams), vp, vp, vp, vp, vp]
     lib.mc_site_energies.argtypes = [vp, f32p]
     lib.mc_local_order.argtypes = [vp, f32p]
     lib.mc_get_directors_aos.argtypes = [vp, f32p]
