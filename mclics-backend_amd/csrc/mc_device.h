@@ -105,10 +105,16 @@ static_assert(offsetof(MCDevState, acc_count) % 8 == 0 && offsetof(MCDevState, b
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11): two 32x32->64 products per round,
 // Weyl key increments between rounds.  (uint64_t)a*b lowers to
-// v_mad_u64_u32, one instruction for both product halves.
+// v_mad_u64_u32, one full-rate instruction for both product halves, and each
+// three-way XOR is one CDNA4 v_bitop3_b32 (truth table 0x96 = a ^ b ^ c),
+// which the compiler does not form by itself from two XORs.
 struct MCU4 {
   uint32_t x, y, z, w;
 };
+
+__device__ __forceinline__ uint32_t mcd_xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 
 __device__ __forceinline__ MCU4 mcd_philox(MCU4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -120,9 +126,9 @@ __device__ __forceinline__ MCU4 mcd_philox(MCU4 c, uint32_t k0, uint32_t k1) {
     uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
     uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
     MCU4 n;
-    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.x = mcd_xor3((uint32_t)(p1 >> 32), c.y, k0);
     n.y = (uint32_t)p1;
-    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.z = mcd_xor3((uint32_t)(p0 >> 32), c.w, k1);
     n.w = (uint32_t)p0;
     c = n;
   }

@@ -33,16 +33,20 @@ __device__ __forceinline__ float sel(const float4 &v, int e) {
 }
 
 // 1 accept, 0 reject, -1 inside the error band (decide exactly).
-// invT = RN(1/T) and epsT = RN(eps/T) come from the pass constants.  The
-// screen's x = -dE*invT differs from -dE/T by at most |x| 2^-23; with the
-// 2^-24 |x| of x*log2(e) that is below the |x| 2^-20 term kept in the band,
-// and the hardware exp2's own error is below the 2^-19 term.
-__device__ __forceinline__ int fast_decide(float dE, float invT, float u, float epsT) {
+// y = -dE/T log2(e) is the screen's exp2 argument, formed as acc * ky with
+// ky = RN(RN(RN(1/T) log2(e)) c) from the pass constants (c the potential's
+// scale of the accumulated bond sum): four roundings in all, as many as the
+// former -RN(dE) RN(1/T) log2(e) chain, so y is within 2^-22 |y| of its exact
+// value and exp2(y) within ln2 |y| 2^-22 = |x| 2^-22 relative (x = y ln 2 =
+// -dE/T), below the |x| 2^-20 term kept in the band; the hardware exp2's own
+// error is below the 2^-19 term of relc = RN(eps/T) + 2^-19.  No range check
+// is needed: for y above 128 the exp2 is +inf and the band test sends the
+// move to the exact path; below -150 it is 0 and u > 0 rejects, as
+// expf(x) = 0 does; in between |x| < 104 keeps rel far below 1.
+__device__ __forceinline__ int fast_decide(float y, float u, float relc) {
   if (u == 0.0f) return -1;
-  const float x = -dE * invT;
-  const float rel = epsT + fabsf(x) * 9.53674316e-7f + 1.9073486e-6f;  // + |x| 2^-20 + 2^-19
-  if (!(rel < 0.125f)) return -1;
-  const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  const float rel = fmaf(fabsf(y), 6.60973460e-7f, relc);  // + |x| 2^-20 = |y| ln2 2^-20
+  const float p = __builtin_amdgcn_exp2f(y);
   if (fabsf(u - p) <= p * rel) return -1;
   return u < p ? 1 : 0;
 }
@@ -91,27 +95,38 @@ __device__ __forceinline__ void exact_energies(const MCDevParams &P, double cA, 
   }
 }
 
-// Proposal of src/evolve.cu:143-163 with a 16-bit axis uniform.
-__device__ __forceinline__ void propose(const float o[3], float av, uint32_t w_angle, uint32_t axis16, float q[3]) {
-  const float va = (2.0f * mcd_u01(w_angle) - 1.0f) * av;
-  const float h = 0.5f * va;  // revolutions: sin(pi*va) = sin(2*pi*h)
+// Proposal of src/evolve.cu:143-163 with a 16-bit axis uniform.  The angle
+// in revolutions, h = pi*va / (2 pi) = 0.5 (2 u0 - 1) ang_var with
+// u0 = (w >> 8) 2^-24, is one fma of the 24-bit integer with the pass
+// constants hs = ang_var 2^-24 and hb = -0.5 ang_var: the exact value of
+// 0.5 (2 u0 - 1) ang_var rounded once, the bits of 0.5 * ((2 u0 - 1) * av).
+__device__ __forceinline__ void propose(const float o[3], float hs, float hb, uint32_t w_angle, uint32_t axis16,
+                                        float q[3]) {
+  const float h = fmaf((float)(w_angle >> 8), hs, hb);  // sin(pi*va) = sin(2*pi*h)
   const float sn = __builtin_amdgcn_sinf(h), cs = __builtin_amdgcn_cosf(h);
   // Rotation about x (u < 0.333), y (u < 0.666) or z, as one rotation of the
   // pair (a, b) = (o1, o2), (o2, o0) or (o0, o1):
   //   x: q1 = o1 cs + o2 sn, q2 = o2 cs - o1 sn
   //   y: q2 = o2 cs + o0 sn, q0 = o0 cs - o2 sn
   //   z: q0 = o0 cs + o1 sn, q1 = o1 cs - o0 sn
-  // Selects instead of branches (the axis differs from lane to lane); the
-  // products and sums are the same operations in the same order.
+  // Selects instead of branches (the axis differs from lane to lane).  The
+  // production proposal need not repeat the reference's roundings (a
+  // recorded sweep hands the oracle the proposals it made), so the products
+  // are fused.
   const bool ax = axis16 < 21824u, ay = !ax && axis16 < 43647u;
   const float a = ax ? o[1] : (ay ? o[2] : o[0]);
   const float b = ax ? o[2] : (ay ? o[0] : o[1]);
-  const float an = a * cs + b * sn;
-  const float bn = b * cs - a * sn;
+  const float an = fmaf(a, cs, b * sn);
+  const float bn = fmaf(b, cs, -(a * sn));
   const float q0 = ax ? o[0] : (ay ? bn : an);
   const float q1 = ax ? an : (ay ? o[1] : bn);
   const float q2 = ax ? bn : (ay ? an : o[2]);
-  const float inv = __builtin_amdgcn_rsqf(q0 * q0 + q1 * q1 + q2 * q2);
+  // Renormalised to first order: |q|^2 = 1 + d with d a few ulps (a rotation
+  // of a unit vector), and q (1.5 - 0.5 |q|^2) has |.|^2 = 1 - 0.75 d^2 + O(d^3),
+  // closer to 1 than a hardware rsq (and a full-rate fma instead of a
+  // transcendental); a director handed in up to 1e-3 off unit length
+  // converges quadratically over its accepted moves.
+  const float inv = fmaf(-0.5f, fmaf(q2, q2, fmaf(q1, q1, q0 * q0)), 1.5f);
   q[0] = q0 * inv;
   q[1] = q1 * inv;
   q[2] = q2 * inv;
@@ -123,7 +138,9 @@ struct PassConst {
   uint32_t sweep;
   double cA;
   float cAf, eps;
-  float invT, epsT;  // 1/T and eps/T for the FP32 screen
+  float ky1, kyLL;  // -RN(1/T) log2(e) and that times cAf: exp2 arguments of dE and of the LL bond sum
+  float relc;       // RN(eps/T) + 2^-19 for the FP32 screen
+  float hs, hb;     // ang_var 2^-24 and -0.5 ang_var (propose)
 };
 
 // Bound on one bond's magnitude from each term's coefficient (unit vectors):
@@ -151,8 +168,11 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
   // with twice the bond bound (screen_bonds)
   const float M = bond_bound(P);
   k.eps = 6.1035156e-5f * ((P.potential == MCD_LL ? M : 2.0f * M) + F);
-  k.invT = 1.0f / k.T;
-  k.epsT = k.eps / k.T;
+  k.ky1 = -((1.0f / k.T) * 1.44269504088896341f);
+  k.kyLL = k.cAf * k.ky1;
+  k.relc = k.eps / k.T + 1.9073486e-6f;
+  k.hs = k.av * 5.96046448e-8f;
+  k.hb = -0.5f * k.av;
   return k;
 }
 
@@ -178,10 +198,12 @@ __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &
     const float e2 = fmaf(s2, S.m[b][2], fmaf(s1, S.m[b][1], s0 * S.m[b][0]));
     acc = fmaf(e1, e2, acc);
   }
-  float dE = k.cAf * acc;
-  if (P.elecA != 0.0f)
-    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
-  return fast_decide(dE, k.invT, S.u, k.epsT);
+  float y = acc * k.kyLL;
+  if (P.elecA != 0.0f) {
+    const float fd = mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
+    y = fmaf(acc, k.kyLL, fd * k.ky1);
+  }
+  return fast_decide(y, S.u, k.relc);
 }
 
 // FP32 screen of one GHRL or pear move: each bond of the reference
@@ -238,7 +260,7 @@ __device__ __forceinline__ int screen_bonds(const MCDevParams &P, const PassCons
   float dE = acc;
   if (P.elecA != 0.0f)
     dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
-  return fast_decide(dE, k.invT, S.u, k.epsT);
+  return fast_decide(dE * k.ky1, S.u, k.relc);
 }
 
 // The neighbours of the pair: yp..zm hold, per component, the +y/-y/+z/-z
@@ -274,8 +296,8 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   }
   const uint32_t ax0 = ((r.x & 0xFFu) << 8) | (r.y & 0xFFu);
   const uint32_t ax1 = ((r.z & 0xFFu) << 8) | (r.w & 0xFFu);
-  propose(S[0].o, k.av, r.x, ax0, S[0].q);
-  propose(S[1].o, k.av, r.z, ax1, S[1].q);
+  propose(S[0].o, k.hs, k.hb, r.x, ax0, S[0].q);
+  propose(S[1].o, k.hs, k.hb, r.z, ax1, S[1].q);
   S[0].u = mcd_u01(r.y);
   S[1].u = mcd_u01(r.w);
   // Screen both sites, then one shared (rarely executed) exact evaluation for

@@ -103,10 +103,11 @@ __device__ __forceinline__ mcsu::Where where_z(const SlabGeom &g, int z) {
 
 // Stores the two sites of parity par (elements par, par+2) of the segment
 // into a half row.
-// (Bitwise select: a plain ?: on the elements is turned into an indexed load
-// from a scratch copy of the segment.)
+// (Bitwise select, one v_bitop3_b32 with the multiplexer table 0xCA =
+// m ? b : a: a plain ?: on the elements is turned into an indexed load from a
+// scratch copy of the segment.)
 __device__ __forceinline__ float pick(float a, float b, uint32_t mask) {
-  return __uint_as_float((__float_as_uint(a) & ~mask) | (__float_as_uint(b) & mask));
+  return __uint_as_float(__builtin_amdgcn_bitop3_b32(mask, __float_as_uint(b), __float_as_uint(a), 0xCA));
 }
 
 __device__ __forceinline__ void put_half(float *slot, const LaneOff &L, int par, const float4 v[3]) {
@@ -230,8 +231,7 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   auto Bbuf = [&](int par) { return lds + (size_t)(par & 1) * H; };
   auto Rbuf = [&](int par) { return lds + (size_t)(2 + (par & 1)) * H; };
 
-  float4 P0[3], P1[3], P2[3], P3[3], Q[3];
-  float4 P4[3] = {};
+  float4 P0[3], P1[3], P2[3], P3[3], P4[3], Q[3];
   uint32_t acc = 0;
 
   // Prologue: planes z0-2 .. z0+2; red updates of z0-1 and z0.
@@ -258,7 +258,11 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   for (int it = 0; it < CZ; ++it) {
     const int p = z0 + it;
     if (active) {
-      if (it <= CZ - 2) load_seg<GEO>(in, g, L, p + 3, P4);
+      // The ring's last slot is loaded in every iteration (plane p+3, and in
+      // the last iteration plane p+2 again, which is not used): a
+      // conditional load would make the compiler re-initialise the slot in
+      // every iteration.
+      load_seg<GEO>(in, g, L, it <= CZ - 2 ? p + 3 : p + 2, P4);
       if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
       if (red_row && has_plane<GEO>(g, p + 1)) {
         acc += update<POT, RECORD, GEO>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
