@@ -43,6 +43,10 @@
 namespace {
 
 constexpr int kMaxThreads = 768;
+// The shape with a compile-time instantiation: 64 segments (256 sites) per
+// row, 8 rows per tile.
+constexpr int kFixedRowSegs = 64;
+constexpr int kFixedTileRows = 8;
 
 using mcsu::sel;
 
@@ -179,17 +183,22 @@ __device__ __forceinline__ uint32_t slab_block_sum(uint32_t v) {
   return t;
 }
 
+// SR, TYC: segments per row and tile rows fixed at compile time (0: read
+// from the parameters).  Fixed values turn the LDS strides into instruction
+// immediates; with SR = 64 a row is one wave, so every row-dependent value
+// (role, parity, plane offsets) is wave-uniform and lives in scalar registers.
+template <int SR = 0, int TYC = 0>
 __device__ __forceinline__ SlabGeom slab_geom(const MCDevParams &P, int TY, int CZ) {
   SlabGeom g;
-  g.X = P.nx;
+  g.X = SR ? 4 * SR : P.nx;
   g.Y = P.ny;
   g.Z = P.nz;
-  g.S = P.nx >> 2;
-  g.TY = TY;
+  g.S = SR ? SR : P.nx >> 2;
+  g.TY = TYC ? TYC : TY;
   g.CZ = CZ;
-  g.R = TY + 4;
-  g.X2 = P.nx >> 1;
-  g.XY = P.nx * P.ny;
+  g.R = g.TY + 4;
+  g.X2 = 2 * g.S;
+  g.XY = g.X * P.ny;
   g.zfree = P.zfree;
   g.zsurf = P.zsurf;
   return g;
@@ -198,7 +207,7 @@ __device__ __forceinline__ SlabGeom slab_geom(const MCDevParams &P, int TY, int 
 // One tile (rows y0.., planes z0..) of one lattice: the whole sweep of the
 // tile described at the top of this file.  Returns this thread's accepted
 // count (owned sites only); every thread of the workgroup must call it.
-template <int POT, bool RECORD, bool GEO>
+template <int POT, bool RECORD, bool GEO, int SR = 0>
 __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::PassConst &kc, const SlabGeom &g,
                                               const float *const in[3], float *const out[3], int y0, int z0,
                                               const MCRecord &rec) {
@@ -210,7 +219,9 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   // short rows (S < 64, several rows to a wave) a wave then updates one
   // parity with one code path, and whole waves skip the updates their rows
   // do not need.  With S = 64 every wave is one row either way.
-  const int q = threadIdx.x % g.S, i = threadIdx.x / g.S, TY = g.R - 4, ne = (TY + 1) >> 1;
+  // (SR = 64: the row index is the wave index, made visibly uniform.)
+  const int q = threadIdx.x % g.S, TY = g.R - 4, ne = (TY + 1) >> 1;
+  const int i = SR == 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : (int)(threadIdx.x / g.S);
   const int r = i < ne ? 2 + 2 * i
                        : (i < TY ? 3 + 2 * (i - ne)
                                  : (i == TY ? 1 : (i == TY + 1 ? g.R - 2 : (i == TY + 2 ? 0 : (i == TY + 3 ? g.R - 1 : i)))));
@@ -289,13 +300,13 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   return acc;
 }
 
-template <int POT, bool RECORD, bool GEO>
+template <int POT, bool RECORD, bool GEO, int SR, int TYC>
 __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const float *__restrict__ inx,
                                                             const float *__restrict__ iny,
                                                             const float *__restrict__ inz, float *__restrict__ outx,
                                                             float *__restrict__ outy, float *__restrict__ outz,
                                                             MCDevState *st, int TY, int CZ, int xcd, MCRecord rec) {
-  const SlabGeom g = slab_geom(P, TY, CZ);
+  const SlabGeom g = slab_geom<SR, TYC>(P, TY, CZ);
   const float *const in[3] = {inx, iny, inz};
   float *const out[3] = {outx, outy, outz};
   const mcsu::PassConst kc = mcsu::pass_const(P, st);
@@ -303,7 +314,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_sweep_slab(MCDevParams P, const
   int tile = blockIdx.x;
   if (xcd && (ntiles & 7) == 0) tile = (blockIdx.x & 7) * (ntiles >> 3) + (blockIdx.x >> 3);
   const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
-  const uint32_t acc = slab_tile<POT, RECORD, GEO>(P, kc, g, in, out, y0, z0, rec);
+  const uint32_t acc = slab_tile<POT, RECORD, GEO, SR>(P, kc, g, in, out, y0, z0, rec);
   const uint32_t bsum = slab_block_sum(acc);
   mcd_finish_sweep(P, st, bsum, true);
 }
@@ -565,16 +576,27 @@ void launch_t(const MCLaunch &L, float *ox, float *oy, float *oz, const MCSlabSh
   const int grid = (L.P.ny / s.ty) * (L.P.nz / s.cz);
   const size_t lds = mc_slab_lds_bytes(L.P, s.ty);
   const MCRecord r = rec ? *rec : MCRecord{};
-#define MC_SLAB(REC_, GEO_)                                                                               \
-  k_sweep_slab<POT, REC_, GEO_><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, s.ty, \
-                                                                  s.cz, s.xcd, r)
+  // Rows of 256 sites (one wave each) in tiles of 8 rows take the
+  // instantiation with that shape fixed at compile time.
+  const bool fixed = L.P.nx == 4 * kFixedRowSegs && s.ty == kFixedTileRows;
+#define MC_SLAB(REC_, GEO_, SR_, TY_)                                                                          \
+  k_sweep_slab<POT, REC_, GEO_, SR_, TY_><<<grid, threads, lds, L.stream>>>(L.P, L.nx, L.ny, L.nz, ox, oy, oz, L.st, \
+                                                                            s.ty, s.cz, s.xcd, r)
+#define MC_SLAB_S(REC_, GEO_)                            \
+  do {                                                   \
+    if (fixed)                                           \
+      MC_SLAB(REC_, GEO_, kFixedRowSegs, kFixedTileRows); \
+    else                                                 \
+      MC_SLAB(REC_, GEO_, 0, 0);                         \
+  } while (0)
   if (L.P.zfree) {
-    if (rec) MC_SLAB(true, true);
-    else MC_SLAB(false, true);
+    if (rec) MC_SLAB_S(true, true);
+    else MC_SLAB_S(false, true);
   } else {
-    if (rec) MC_SLAB(true, false);
-    else MC_SLAB(false, false);
+    if (rec) MC_SLAB_S(true, false);
+    else MC_SLAB_S(false, false);
   }
+#undef MC_SLAB_S
 #undef MC_SLAB
 }
 
@@ -682,18 +704,30 @@ int mc_slab_prepare(size_t lds_bytes) {
       reinterpret_cast<const void *>(&k_batch_slab<MCD_LL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_GHRL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_PEAR>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, false>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, true>),
-      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, true>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, true, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, true, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, false, true, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_GHRL, true, true, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, false, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, false, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, false, true, kFixedRowSegs, kFixedTileRows>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, true, 0, 0>),
+      reinterpret_cast<const void *>(&k_sweep_slab<MCD_PEAR, true, true, kFixedRowSegs, kFixedTileRows>),
       reinterpret_cast<const void *>(&k_measure_slab<MCD_LL, false>),
       reinterpret_cast<const void *>(&k_measure_slab<MCD_GHRL, false>),
       reinterpret_cast<const void *>(&k_measure_slab<MCD_PEAR, false>),

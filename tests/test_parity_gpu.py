@@ -373,6 +373,64 @@ def test_segment_kernel_replay(pot, T, field, kernel):
     np.testing.assert_allclose(np.linalg.norm(rec["prop"].reshape(-1, 3), axis=1), 1.0, atol=3e-7)
 
 
+@pytest.mark.parametrize("pot,T,geo", [(O.LL, 1.13, False), (O.LL, 1e-4, False), (O.LL, 40.0, False),
+                                       (O.GHRL, 0.5, True), (O.PEAR, 0.8, False), (O.LL, 0.9, True)])
+def test_slab_fixed_shape_replay(pot, T, geo):
+    """Rows of 256 sites in tiles of 8 rows take the slab kernel's
+    compile-time instantiation (a row is one wave).  Its recorded sweeps must
+    replay exactly in the oracle, on a periodic lattice and on the slab
+    geometry (free z, homeotropic GHRL faces, field), with the tiles meeting
+    inside the lattice on y and z."""
+    rng = np.random.default_rng(int(T * 1000) + 7 * pot + geo)
+    nx, ny, nz = 256, 16, 10
+    nt = nx * ny * nz
+    kw = {"k": GHRL_K} if pot == O.GHRL else ({"A": 1.0, "B1": 0.3, "Cc": 0.3} if pot == O.PEAR else {})
+    surf, pt, ns, bound = [], np.ones(nt, np.int32), None, (1, 1, 1)
+    if geo:
+        bound = (1, 1, 0)
+        kw.update(elec=(0.0, 0.0, 1.0), elecA=1.0, elecE=0.5)
+        surf = [("homeotropic_ghrl" if pot == O.GHRL else "homeotropic", 5.0, (0, 0, 1)), ("rp", 2.0, (1, 0, 0))]
+        ns = np.zeros(3 * nt, np.float32)
+        for j in range(ny):
+            for i in range(nx):
+                s0, s1 = i + nx * j, i + nx * (j + ny * (nz - 1))
+                pt[s0], pt[s1] = 2, 3
+                ns[3 * s0 + 2], ns[3 * s1 + 2] = -1.0, 1.0
+    po = O.params(nx, ny, nz, bound=bound, potential=pot, surfaces=surf, **kw)
+    cfg = mclics.Config(nx, ny, nz, bound=bound, potential=pot, A=po.A, B1=po.B1, C=po.C, ghrl=O.ghrl_tuple(po),
+                        elec=(po.elecX, po.elecY, po.elecZ), elecA=po.elecA, elecE=po.elecE,
+                        surfaces=[mclics.Surface(k, w, d) for k, w, d in surf], T=float(np.float32(T)), seed=77)
+    n = unit(rng, nt)
+    ref = n.copy()
+    with mclics.Lattice(cfg, n, pt, ns) as lat:
+        lat.set_kernel(mclics.KERNEL_SLAB)
+        lat.set_slab_tile(8, 5)  # 2 row tiles x 2 plane tiles
+        for w in range(3):
+            av = lat.ang_var()
+            rec = lat.record_sweep()
+            total = 0
+            for colour in (0, 1):
+                cnt, racc, reo, ren = O.pass_inject(po, ref, pt, ns, 8 + colour, np.float32(T), rec["prop"],
+                                                    rec["u2"])
+                sel = in_set(nx, ny, nz, 8 + colour)
+                assert np.array_equal(rec["accept"][sel], racc[sel]), (w, colour)
+                assert np.array_equal(rec["e_old"][sel], reo[sel]), (w, colour)
+                assert np.array_equal(rec["e_new"][sel], ren[sel]), (w, colour)
+                total += cnt
+            assert np.array_equal(lat.directors(), ref), w
+            assert lat.acceptance() == (total, w + 1)
+            assert lat.ang_var() == O.lib().orc_adapt_ang_var(av, total, nt)
+        # the production instantiation of the same shape continues the same chain as the segment kernel
+        with mclics.Lattice(cfg, ref, pt, ns) as seg:
+            seg.set_kernel(mclics.KERNEL_X4)
+            seg.set_sweep_index(3)
+            seg.set_ang_var(lat.ang_var())
+            lat.sweeps(2)
+            seg.sweeps(2)
+            assert np.array_equal(lat.directors(), seg.directors())
+            assert lat.acceptance() == seg.acceptance()
+
+
 @pytest.mark.parametrize("dims,pot,zfree,faces,kw", [
     ((32, 16, 16), O.LL, False, False, {"A": 1.0}),
     ((32, 16, 12), O.GHRL, True, True, {"k": GHRL_K, "elec": (0.0, 0.0, 1.0), "elecA": 1.0, "elecE": 0.5}),
