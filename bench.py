@@ -161,7 +161,7 @@ def batch_leg(n_sweeps=400):
 
 def float_loop(xi, xf, dx):
     """The reference's float loop, for (x = xi; (int)1e6*sign*(x - xf) >= 0;
-    x += dx) in FP32 with sign = (int)(-dx/|dx|) (src/evolve_thermal.cu:77)."""
+    x += dx) in FP32 with sign = (int)(-dx/|dx|) (src/evolve_thermal.cu:69,79)."""
     xs, x = [], np.float32(xi)
     sign = int(-dx / abs(dx))
     while np.float32(1000000 * sign) * (x - np.float32(xf)) >= 0:

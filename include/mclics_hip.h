@@ -72,7 +72,12 @@ typedef struct mc_params {
   int32_t n_surf;
   int32_t surf_kind[MC_MAX_SURF]; /* mc_anchoring */
   float surf_W[MC_MAX_SURF];
-  float surf_dir[MC_MAX_SURF][3]; /* (theta_s, phi_s) direction, rp/strong kinds */
+  /* Anchoring direction of the rp, strong, rp_ghrl and strong_ghrl kinds: the
+   * UNIT VECTOR (sin t cos p, sin t sin p, cos t) of the surface's
+   * (theta_s, phi_s) = (t, p) in degrees (src/anchoring_rp.cpp:42-45), not
+   * the angles; mc_create rejects a direction of these kinds whose length is
+   * not 1 (within 1e-3). */
+  float surf_dir[MC_MAX_SURF][3];
   uint64_t seed;                  /* keys the counter-based generator */
   uint32_t stream;                /* independent stream id under one seed */
   int32_t order;                  /* mc_order */
@@ -86,8 +91,8 @@ const char *mc_last_error(void);
 int mc_device_count(void);
 
 /* Replaces the device allocation and upload done at the start of every
- * reference run(): src/evolve_thermal.cu:45-64 (and step:42-63,
- * quench:157-178, electric:279-299).  ns may be NULL when no site has
+ * reference run(): src/evolve_thermal.cu:45-64 (and src/evolve_step.cu:42-64,
+ * src/evolve_quench.cu:42-64, src/evolve_electric.cu:45-65).  ns may be NULL when no site has
  * pt >= 2. */
 int mc_create(const mc_params *params, const float *ni_aos, const int32_t *pt, const float *ns_aos,
               mc_handle **out);

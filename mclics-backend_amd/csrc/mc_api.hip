@@ -413,9 +413,18 @@ int mc_create(const mc_params *p, const float *ni_aos, const int32_t *pt, const 
   }
   if (p->potential < MC_LL || p->potential > MC_PEAR) return fail(MC_EINVAL, "mc_create: unknown potential");
   if (p->n_surf < 0 || p->n_surf > MC_MAX_SURF) return fail(MC_EINVAL, "mc_create: n_surf out of range");
-  for (int s = 0; s < p->n_surf; ++s)
+  for (int s = 0; s < p->n_surf; ++s) {
     if (p->surf_kind[s] < MC_HOMEOTROPIC || p->surf_kind[s] > MC_STRONG_GHRL)
       return fail(MC_EINVAL, "mc_create: unknown anchoring kind");
+    const int k = p->surf_kind[s];
+    if (k == MC_RP || k == MC_STRONG || k == MC_RP_GHRL || k == MC_STRONG_GHRL) {
+      const float *d = p->surf_dir[s];
+      const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!isfinite(d2) || fabsf(d2 - 1.0f) > 1e-3f)
+        return fail(MC_EINVAL, "mc_create: surf_dir of surface " + std::to_string(s) +
+                                   " must be the unit vector of (theta_s, phi_s), not the angles");
+    }
+  }
   if (p->order != MC_ORDER_CHECKER && p->order != MC_ORDER_REF8)
     return fail(MC_EINVAL, "mc_create: unknown sweep order");
   if (!(p->T > 0.0f) || !isfinite(p->T)) return fail(MC_EINVAL, "mc_create: temperature must be finite and > 0");

@@ -178,14 +178,14 @@ __device__ __forceinline__ float mcd_dot(float a0, float a1, float a2, float b0,
   return t;
 }
 
-// src/potential.cpp:210-213: -A*1.5*nij*nij evaluated in double.
+// src/potential.cpp:17-20: -A*1.5*nij*nij evaluated in double.
 __device__ __forceinline__ float mcd_bond_ll(double cA, float nij) {
   double v = cA * (double)nij;
   v = v * (double)nij;
   return (float)v;
 }
 
-// src/potential.cpp:215-231 (CPU form: E1 built in double, em*(...-1.0/9) in
+// src/potential.cpp:22-38 (CPU form: E1 built in double, em*(...-1.0/9) in
 // double, the sum in double).
 __device__ __forceinline__ float mcd_bond_ghrl(const MCDevParams &P, float n0, float n1, float n2, float m0,
                                                float m1, float m2, float r0, float r1, float r2) {
@@ -213,7 +213,7 @@ __device__ __forceinline__ float mcd_bond_ghrl(const MCDevParams &P, float n0, f
   return (float)s;
 }
 
-// src/potential.cpp:203-208.
+// src/potential.cpp:10-15.
 __device__ __forceinline__ float mcd_bond_pear(const MCDevParams &P, float n0, float n1, float n2, float m0,
                                                float m1, float m2, float r0, float r1, float r2) {
   float nij = mcd_dot(n0, n1, n2, m0, m1, m2);
@@ -229,7 +229,7 @@ __device__ __forceinline__ float mcd_bond_pear(const MCDevParams &P, float n0, f
   return (float)v;
 }
 
-// src/potential.cpp:232-236.
+// src/potential.cpp:39-43.
 __device__ __forceinline__ float mcd_field(const MCDevParams &P, float elecE, float n0, float n1, float n2) {
   float d = n0 * P.eX;
   d = d + n1 * P.eY;

@@ -20,7 +20,7 @@ constexpr int kBlock = MCD_BLOCK;
 
 // Neighbour indices in the reference order +x,-x,+y,-y,+z,-z and whether each
 // bond exists: periodic axes wrap, free axes drop out-of-range neighbours
-// (src/potential.cpp:237-247), inactive neighbours drop out (pt==0).
+// (src/potential.cpp:44-54), inactive neighbours drop out (pt==0).
 template <bool GEN>
 __device__ __forceinline__ void mcd_neighbours(const MCDevParams &P, const uint8_t *pt, int i, int j, int k,
                                                int nb[6], bool on[6]) {
@@ -49,7 +49,7 @@ __device__ __forceinline__ void mcd_neighbours(const MCDevParams &P, const uint8
 }
 
 // Old (and optionally new) site energy of site s, the reference's
-// latice_Potential (src/geometry_bulk.cpp:63-72, src/geometry_slab.cpp:141-155).
+// latice_Potential (src/geometry_bulk.cpp:63-72, src/geometry_slab.cpp:69-83).
 template <int POT, bool GEN, bool TWO>
 __device__ __forceinline__ void mcd_site_energy(const MCDevParams &P, double cA, const float *__restrict__ nx,
                                                 const float *__restrict__ ny, const float *__restrict__ nz,

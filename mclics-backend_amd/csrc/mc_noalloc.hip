@@ -20,6 +20,7 @@
 // workgroup writes its whole row of partials and the finisher reads only the
 // rows of the current launch; the state is written by the call itself.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <string.h>
 
 #include <mutex>
@@ -120,9 +121,15 @@ extern "C" int mc_measure_noalloc(const mc_params *p, const float *nx, const flo
     return fail(MC_EINVAL, "mc_measure_noalloc: a periodic z needs even Nz and no surfaces");
   if (p->bound[2] == MC_FREE && (p->nz < 2 || (p->n_surf != 0 && p->n_surf < 2) || p->n_surf > MC_MAX_SURF))
     return fail(MC_EINVAL, "mc_measure_noalloc: a free z takes no surfaces or the slab faces (surfaces 0 and 1)");
-  for (int s = 0; s < p->n_surf; ++s)
+  for (int s = 0; s < p->n_surf; ++s) {
     if (p->surf_kind[s] < MC_HOMEOTROPIC || p->surf_kind[s] > MC_STRONG_GHRL)
       return fail(MC_EINVAL, "mc_measure_noalloc: unknown anchoring kind");
+    const int k = p->surf_kind[s];
+    const float *d = p->surf_dir[s];
+    const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if ((k == MC_RP || k == MC_STRONG || k == MC_RP_GHRL || k == MC_STRONG_GHRL) && !(fabsf(d2 - 1.0f) <= 1e-3f))
+      return fail(MC_EINVAL, "mc_measure_noalloc: surf_dir must be a unit vector for the rp and strong kinds");
+  }
 
   DeviceSlot slot;
   int rc = device_slot(&slot);

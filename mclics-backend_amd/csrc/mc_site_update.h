@@ -4,7 +4,7 @@
 // A thread owns a 16-byte x segment [4q, 4q+4) of one row; the two sites of
 // the colour being updated sit at elements PAR and PAR+2.  The caller gathers
 // the six neighbours of both sites (bond order +x, -x, +y, -y, +z, -z, the
-// order of src/evolve.cpp:128-142) and this header proposes, decides and
+// order of src/evolve.cpp:119-129) and this header proposes, decides and
 // returns the two decisions.
 //
 // Randomness.  One Philox4x32-10 block per pair (counter = (index of the first
@@ -207,7 +207,7 @@ __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &
 }
 
 // FP32 screen of one GHRL or pear move: each bond of the reference
-// (src/potential.cpp:203-231) with every FP64 step done in FP32.  n.r, m.r
+// (src/potential.cpp:10-38) with every FP64 step done in FP32.  n.r, m.r
 // and (n x m).r are read off as signed components (r is a lattice axis), the
 // rest is the reference's FP32 arithmetic.  Error (u = 2^-24, unit vectors,
 // M = bond_bound): the reference's n_ij, P2, cross, nu P2 and sigma sg cross

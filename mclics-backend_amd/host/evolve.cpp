@@ -193,7 +193,7 @@ double EvolveNGPU::polarization() const {
   return sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
 }
 
-// src/evolve_thermal.cu:33-116 (CPU twin src/evolve_thermal.cpp:146-211).
+// src/evolve_thermal.cu:33-116 (CPU twin src/evolve_thermal.cpp:30-95).
 int thermalEvolveNGPU::run() {
   params->T = params->Ti;
   open();
@@ -235,7 +235,7 @@ int stepEvolveNGPU::run() {
   return 0;
 }
 
-// src/evolve_quench.cu:146-234: MCT sweeps at Ti, then (int)(MCT*dT) at Tf
+// src/evolve_quench.cu:31-119: MCT sweeps at Ti, then (int)(MCT*dT) at Tf
 // (the CPU build runs every integer below MCT*dT, src/evolve_quench.cpp:62).
 int quenchEvolveNGPU::run() {
   params->T = params->Ti;
@@ -261,7 +261,7 @@ int quenchEvolveNGPU::run() {
   return 0;
 }
 
-// src/evolve_electric.cu:267-352.
+// src/evolve_electric.cu:33-118.
 int electricEvolveNGPU::run() {
   params->T = params->Ti;
   params->elecE = params->elecEi;

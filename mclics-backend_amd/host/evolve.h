@@ -76,7 +76,7 @@ class electricEvolveNGPU : public EvolveNGPU {
   int run() override;
 };
 
-// Writes a director snapshot (src/io.cpp:205-235): "x,y,z,nx,ny,nz,S,pt"
+// Writes a director snapshot (src/io.cpp:50-80): "x,y,z,nx,ny,nz,S,pt"
 // with the local order parameter of each active site and NaN elsewhere.
 // S_site, when given, holds those values (from mc_local_order); otherwise
 // they are computed here.

@@ -22,7 +22,7 @@ struct KindInfo {
   bool needs_angles;
 };
 
-// src/geometry.cpp:86-105 and the name[] members of include/anchoring.h.
+// src/geometry.cpp:82-97 and the name[] members of include/anchoring.h.
 const KindInfo kKinds[] = {
     {"rp", MC_RP, "Rapine Papoular Anchoring", true},
     {"fg", MC_FG, "Founier-Galatola like Anchoring", false},
@@ -88,7 +88,7 @@ void Geometry::Boundary_Init(Parameters *p) {
       if (ph == p->phi_s.end()) missing_parameter("phi_s", id, info->cls);
       auto th = p->theta_s.find(id);
       if (th == p->theta_s.end()) missing_parameter("theta_s", id, info->cls);
-      // src/anchoring_rp.cpp:136-139: FP32 degrees-to-radians and float trig.
+      // src/anchoring_rp.cpp:42-45: FP32 degrees-to-radians and float trig.
       const float toPi = (float)(M_PI / 180);
       a.n_s[0] = cosf(toPi * ph->second) * sinf(toPi * th->second);
       a.n_s[1] = sinf(toPi * ph->second) * sinf(toPi * th->second);
@@ -110,7 +110,7 @@ Bulk_Geometry::Bulk_Geometry(int *pt, Parameters *p) : Geometry(p) {
   resolve_bounds(p, true, true, true);
 }
 
-// src/geometry_slab.cpp:119-139: anchoring planes at z=0 (normal -z, pt 2)
+// src/geometry_slab.cpp:47-67: anchoring planes at z=0 (normal -z, pt 2)
 // and z=Nz-1 (normal +z, pt 3); z is forced free.
 Slab_Geometry::Slab_Geometry(int *pt, Parameters *p) : Geometry(p) {
   print_field("Geometry:", "Slab");

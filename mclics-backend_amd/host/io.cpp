@@ -1,6 +1,6 @@
 // Parameter file parser and console helpers of the host drop-in.
 //
-// The parser reproduces the reference's stream semantics (src/io.cpp:236-408)
+// The parser reproduces the reference's stream semantics (src/io.cpp:81-253)
 // because drop-in users rely on them: values are read with operator>> into
 // the field's own type, the rest of the line is discarded with getline, and
 // the stream state is checked after every key exactly as check_error_bits

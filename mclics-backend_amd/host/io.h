@@ -8,7 +8,7 @@
 
 namespace mc {
 
-// read_input_file (src/io.cpp:236-408): whitespace-separated "key value"
+// read_input_file (src/io.cpp:81-253): whitespace-separated "key value"
 // tokens, case-insensitive keys with the reference's aliases, '#' starts a
 // comment that runs to the end of the line, anything after a value on its
 // line is ignored, surface keys take an index first ("W 0 4").  Errors throw

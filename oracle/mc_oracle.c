@@ -23,14 +23,14 @@
 #endif
 
 static inline float dot3f(const float a[3], const float b[3]) {
-  /* ((a0*b0 + a1*b1) + a2*b2) in FP32, as written in src/potential.cpp:211. */
+  /* ((a0*b0 + a1*b1) + a2*b2) in FP32, as written in src/potential.cpp:18. */
   float t = a[0] * b[0];
   t = t + a[1] * b[1];
   t = t + a[2] * b[2];
   return t;
 }
 
-/* src/potential.cpp:210-213: -params->A * 1.5 * nij * nij, the 1.5 a double
+/* src/potential.cpp:17-20: -params->A * 1.5 * nij * nij, the 1.5 a double
  * literal, so the chain is evaluated in double and rounded to float once. */
 static float bond_ll(const orc_params *p, const float ni[3], const float nj[3]) {
   float nij = dot3f(ni, nj);
@@ -40,7 +40,7 @@ static float bond_ll(const orc_params *p, const float ni[3], const float nj[3]) 
   return (float)v;
 }
 
-/* src/potential.cpp:215-231.  Types: E1 is a float holding a double
+/* src/potential.cpp:22-38.  Types: E1 is a float holding a double
  * expression; em*(ai*aj*nij - 1.0/9) is double; the whole sum is double and
  * rounded to float on return. */
 static float bond_ghrl(const orc_params *p, const float ni[3], const float nj[3], const float r[3]) {
@@ -69,7 +69,7 @@ static float bond_ghrl(const orc_params *p, const float ni[3], const float nj[3]
   return (float)s;
 }
 
-/* src/potential.cpp:203-208.  (C*0.25) is double, so the last subtraction is
+/* src/potential.cpp:10-15.  (C*0.25) is double, so the last subtraction is
  * done in double. */
 static float bond_pear(const orc_params *p, const float ni[3], const float nj[3], const float r[3]) {
   float nij = dot3f(ni, nj);
@@ -96,7 +96,7 @@ float orc_bond(const orc_params *p, const float ni[3], const float nj[3], const 
   }
 }
 
-/* src/potential.cpp:232-236. */
+/* src/potential.cpp:39-43. */
 float orc_field(const orc_params *p, const float ni[3]) {
   float d = ni[0] * p->elecX;
   d = d + ni[1] * p->elecY;
@@ -105,8 +105,8 @@ float orc_field(const orc_params *p, const float ni[3]) {
   return -p->elecA * (nde * nde);
 }
 
-/* Shared GHRL surface form of src/anchoring_homeotropic.cpp:77-94,
- * src/anchoring_rp.cpp:171-192 and src/anchoring_fg.cpp:243-266, all in FP32;
+/* Shared GHRL surface form of src/anchoring_homeotropic.cpp:51-68,
+ * src/anchoring_rp.cpp:77-98 and src/anchoring_fg.cpp:51-74, all in FP32;
  * the (1/9) there is integer division and contributes 0. */
 static float ghrl_surface_form(const orc_params *p, float W, const float ni[3], const float nj[3],
                                const float s[3], float ai, float aj) {
@@ -135,17 +135,17 @@ float orc_surface(const orc_params *p, int surf, const float ni[3], const float 
   const float W = p->surf_W[surf];
   const float *d = p->surf_dir[surf];
   switch (p->surf_kind[surf]) {
-    case ORC_HOMEOTROPIC: { /* src/anchoring_homeotropic.cpp:55-60 */
+    case ORC_HOMEOTROPIC: { /* src/anchoring_homeotropic.cpp:29-34 */
       float nij = dot3f(ni, s);
       return -W * nij * nij;
     }
-    case ORC_RP:     /* src/anchoring_rp.cpp:135-142 */
-    case ORC_STRONG: /* src/anchoring_strong.cpp:307-314 */
-    case ORC_STRONG_GHRL: { /* src/anchoring_strong.cpp:343-350 */
+    case ORC_RP:     /* src/anchoring_rp.cpp:41-48 */
+    case ORC_STRONG: /* src/anchoring_strong.cpp:41-48 */
+    case ORC_STRONG_GHRL: { /* src/anchoring_strong.cpp:77-84 */
       float nij = dot3f(ni, d);
       return -W * nij * nij;
     }
-    case ORC_FG: { /* src/anchoring_fg.cpp:221-226 */
+    case ORC_FG: { /* src/anchoring_fg.cpp:29-34 */
       float nij = dot3f(ni, s);
       return +W * nij * nij;
     }
@@ -158,7 +158,7 @@ float orc_surface(const orc_params *p, int surf, const float ni[3], const float 
       float aj = dot3f(d, s);
       return ghrl_surface_form(p, W, ni, d, s, ai, aj);
     }
-    case ORC_FG_GHRL: { /* src/anchoring_fg.cpp:251-256: projected partner */
+    case ORC_FG_GHRL: { /* src/anchoring_fg.cpp:59-64: projected partner */
       float mod = sqrtf(fabsf(dot3f(ni, s)));
       float nj[3] = {ni[0] - s[0] * mod, ni[1] - s[1] * mod, ni[2] - s[2] * mod};
       mod = sqrtf(fabsf(dot3f(nj, nj)));
@@ -185,7 +185,7 @@ void orc_set_ghrl(orc_params *p, float k11, float k22, float k33, float p0) {
   p->ghrl_nu = (Scale / 9) * (k11 - 3 * k22 - k33);
 }
 
-/* Boundary resolution, src/potential.cpp:237-247.  Returns 1 and wraps *ii
+/* Boundary resolution, src/potential.cpp:44-54.  Returns 1 and wraps *ii
  * for an existing neighbour, 0 when a free boundary drops it. */
 static inline int resolve(int bound, int *ii, int N) {
   if (bound == ORC_PERIODIC) {
@@ -495,14 +495,6 @@ double orc_local_S(const orc_params *p, const float *n, const int32_t *pt, int i
   double ev[3];
   orc_sym3_eigvals(q6, ev);
   return ev[0];
-}
-
-double orc_polarization(const float *n, int64_t nt) {
-  double P[3] = {0, 0, 0};
-  for (int64_t s = 0; s < nt; ++s)
-    for (int c = 0; c < 3; ++c) P[c] += n[3 * s + c];
-  for (int c = 0; c < 3; ++c) P[c] /= (double)nt;
-  return sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
 }
 
 int orc_float_loop(float xi, float xf, float dx, float *xs, int max_rows) {

@@ -29,7 +29,7 @@ extern "C" {
 
 enum { ORC_FREE = 0, ORC_PERIODIC = 1 };
 enum { ORC_LL = 0, ORC_GHRL = 1, ORC_PEAR = 2 };
-/* Anchoring kinds, src/geometry.cpp:86-105. */
+/* Anchoring kinds, src/geometry.cpp:82-97. */
 enum {
   ORC_HOMEOTROPIC = 0,
   ORC_RP = 1,
@@ -105,8 +105,6 @@ void orc_sym3_eigvals(const double q6[6], double ev[3]);
 /* Local order parameter of a site and its active von Neumann neighbours
  * (src/parameter_order.cpp:62-113), FP64. */
 double orc_local_S(const orc_params *p, const float *n, const int32_t *pt, int i, int j, int k);
-/* Polarisation |<n>| over all sites (src/parameter_order.cpp:142-155). */
-double orc_polarization(const float *n, int64_t nt);
 
 /* Number of rows of the float loop for(x=xi; (int)1e6*sign*(x-xf)>=0; x+=dx),
  * sign = (int)(-dx/|dx|) (src/evolve_thermal.cu:69,79).  Writes up to

@@ -77,8 +77,6 @@ def lib():
         L.orc_sym3_eigvals.argtypes = [dp, dp]
         L.orc_local_S.argtypes = [P, f32p, i32p, C.c_int, C.c_int, C.c_int]
         L.orc_local_S.restype = C.c_double
-        L.orc_polarization.argtypes = [f32p, C.c_int64]
-        L.orc_polarization.restype = C.c_double
         L.orc_float_loop.argtypes = [C.c_float, C.c_float, C.c_float, f32p, C.c_int]
         L.orc_float_loop.restype = C.c_int
         L.orc_expf_n.argtypes = [f32p, f32p, C.c_int64]

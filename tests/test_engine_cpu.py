@@ -67,7 +67,7 @@ def test_params_struct_layout_matches_header():
     assert engine.McParams.ang_var.offset == off_av
 
 
-@pytest.mark.parametrize("bad", ["odd_periodic", "non_unit", "nan", "bad_pt", "zero_T", "no_active"])
+@pytest.mark.parametrize("bad", ["odd_periodic", "non_unit", "nan", "bad_pt", "zero_T", "no_active", "surf_dir_angles"])
 def test_create_validates_before_device(bad):
     _ensure_built()
     nx, ny, nz = 4, 4, 4
@@ -89,9 +89,14 @@ def test_create_validates_before_device(bad):
         cfg.T = 0.0
     elif bad == "no_active":
         pt[:] = 0
+    elif bad == "surf_dir_angles":
+        # an rp direction handed in as (theta_s, phi_s) degrees instead of the unit vector
+        cfg.surfaces = [mclics.Surface("rp", 2.0, (90.0, 0.0, 0.0))]
     with pytest.raises(mclics.EngineError) as ei:
         mclics.Lattice(cfg, n, pt)
     assert ei.value.code == -1
+    if bad == "surf_dir_angles":
+        assert "unit vector" in str(ei.value)
 
 
 @pytest.mark.parametrize("xi,xf,dx", [(1.2, 0.0, -0.02), (0.0, 1.0, 0.05), (1.2, 1.0, -0.1), (0.5, 0.5, 0.1),

@@ -214,6 +214,58 @@ def test_surface_kinds_closed_forms():
     assert O.surface(p, 6, ez, s) == pytest.approx(+1.0)
 
 
+@pytest.mark.parametrize("p0", [0.0, 16.0])
+def test_ghrl_anchoring_closed_forms_unequal_constants(p0):
+    # The three GHRL surface forms at k = (1.0, 0.7, 1.3), where every GHRL
+    # coefficient is nonzero (S = 3.75: rho = -0.125, lambda = 0.5,
+    # mu = -1.125, nu = -1; sigma = 0 or -3.75*0.7*2pi/16), with the site
+    # normal s = z and the rp direction d = x (theta_s = 90, phi_s = 0).
+    # Values worked out by hand from src/anchoring_homeotropic.cpp:51-68,
+    # src/anchoring_rp.cpp:77-98 and src/anchoring_fg.cpp:51-74
+    # (E = W [E1 (rho P2 + lambda) + mu a_i a_j n_ij - (1/9 in integers = 0)
+    #  + nu P2 + sigma sgn(n_ij) cross], cross = (n_i x n_j).(-s)).
+    W = 2.5
+    rho, lam, mu, nu = -0.125, 0.5, -1.125, -1.0
+    sig = -3.75 * 0.7 * 2 * math.pi / 16 if p0 else 0.0
+    kinds = [("homeotropic_ghrl", W, (1, 0, 0)), ("rp_ghrl", W, (1, 0, 0)), ("fg_ghrl", W, (1, 0, 0))]
+    p = O.params(2, 2, 2, potential=O.GHRL, k=(1.0, 0.7, 1.3, p0), surfaces=kinds)
+    assert O.ghrl_tuple(p) == pytest.approx((rho, lam, mu, nu, sig), rel=1e-6)
+    s = [0.0, 0.0, 1.0]
+    c = math.sqrt(0.5)
+    E = lambda kind, n: O.surface(p, kind, n, s)
+    # homeotropic_ghrl: partner and bond direction are s, so a_j = 1, n_ij = a_i = n.s
+    # and the chiral term vanishes ((n x s).s = 0) for either p0.
+    assert E(0, [0, 0, 1]) == pytest.approx(W * (2 * (rho + lam) + mu + nu), rel=1e-6)   # -1.375 W
+    assert E(0, [0, 0, 1]) == pytest.approx(-1.375 * W, rel=1e-6)
+    assert E(0, [1, 0, 0]) == pytest.approx(0.78125 * W, rel=1e-6)                      # a = 0
+    assert E(0, [c, 0, c]) == pytest.approx(-0.2265625 * W, rel=1e-6)                   # a = 1/sqrt2
+    for a in (0.1, 0.35, 0.8, -0.6):
+        n = [math.sqrt(1 - a * a), 0.0, a]
+        P2 = 1.5 * a * a - 0.5
+        want = W * ((1.5 * a * a + 0.5) * (rho * P2 + lam) + mu * a * a + nu * P2)
+        assert E(0, n) == pytest.approx(want, rel=2e-6, abs=2e-6)
+    # rp_ghrl: partner d = x, bond direction s; a_j = d.s = 0
+    assert E(1, [1, 0, 0]) == pytest.approx(-1.375 * W, rel=1e-6)                       # n = d
+    assert E(1, [0, 0, 1]) == pytest.approx(0.78125 * W, rel=1e-6)                      # n = s
+    # n = y: n_ij = 0 (sgn -1), cross = 1: W (-(lambda - rho/2) - nu/2 - sigma)
+    assert E(1, [0, 1, 0]) == pytest.approx(W * (-0.0625 - sig), rel=1e-6, abs=1e-6)
+    # n = (c, +-c, 0): n_ij = c, cross = +-c
+    assert E(1, [c, c, 0]) == pytest.approx(W * (-0.71875 + sig * c), rel=1e-6, abs=2e-6)
+    assert E(1, [c, -c, 0]) == pytest.approx(W * (-0.71875 - sig * c), rel=1e-6, abs=2e-6)
+    assert E(1, [-c, c, 0]) == pytest.approx(W * (-0.71875 - sig * c), rel=1e-6, abs=2e-6)       # n_ij < 0 flips sgn
+    # fg_ghrl: partner = n - s sqrt|n.s| normalised (in the n, s plane: no chiral term)
+    assert E(2, [0, 0, 1]) == pytest.approx(0.78125 * W, rel=1e-6)                      # partner vanishes
+    assert E(2, [1, 0, 0]) == pytest.approx(-1.375 * W, rel=1e-6)                       # partner = n
+    m = math.sqrt(c)
+    nj = [c, 0.0, c - m]
+    nn = math.sqrt(nj[0] ** 2 + nj[2] ** 2)
+    nj = [nj[0] / nn, 0.0, nj[2] / nn]
+    ai, aj, nij = c, nj[2], c * nj[0] + c * nj[2]
+    P2 = 1.5 * nij * nij - 0.5
+    want = W * ((1.5 * ai * ai + 1.5 * aj * aj - 1) * (rho * P2 + lam) + mu * ai * aj * nij + nu * P2)
+    assert E(2, [c, 0, c]) == pytest.approx(want, rel=2e-6, abs=3e-6)  # terms of ~0.5 cancel to 0.02 in FP32
+
+
 # --------------------------------------------------------------------------
 # Accept rule, ordering and adaptation.
 

@@ -1,6 +1,6 @@
 """End-to-end runs of the drop-in executable (mc_sim) on the GPU: the *GPU
 modes on small lattices, checked for the reference's po.dat columns and row
-counts (float loops, src/evolve_thermal.cu:77, src/evolve_electric.cu:80),
+counts (float loops, src/evolve_thermal.cu:79, src/evolve_electric.cu:81),
 snapshot names ((int)(100*(T+1e-5)), (int)(1000*(E+1e-5)), block index) and
 format, and the snapshot's S column against the oracle's local order
 parameter of the snapshot's own directors; plus the device local order
