@@ -131,7 +131,10 @@ int mc_autotune(mc_handle *h, int *grid, int *xcd, float *best_us);
  * asynchronous on the handle's stream, with the ang_var adaptation done on
  * the device after each sweep so no host round trip is needed. */
 int mc_sweeps(mc_handle *h, int n);
-/* Same, captured once into a hipGraph per distinct n and replayed. */
+/* Same, through captured hipGraphs of at most 64 sweeps each: the graph of
+ * 64 sweeps is replayed n / 64 times, then the graph of the n % 64 that
+ * remain.  Graphs are captured on first use and cached per handle (at most
+ * 128 of them); the chain equals mc_sweeps's bit for bit. */
 int mc_sweeps_graph(mc_handle *h, int n);
 /* Blocks until queued work is done. */
 int mc_sync(mc_handle *h);
@@ -232,6 +235,7 @@ typedef struct mc_batch mc_batch;
 int mc_batch_create(int nrep, const mc_params *params, const float *const *ni_aos, mc_batch **out);
 int mc_batch_destroy(mc_batch *b);
 int mc_batch_set_T(mc_batch *b, int rep, float T);
+/* n sweeps of every replica, as graphs of at most 64 sweeps (as mc_sweeps_graph). */
 int mc_batch_sweeps(mc_batch *b, int n);
 int mc_batch_sync(mc_batch *b);
 int mc_batch_time_sweeps(mc_batch *b, int n, float *ms);

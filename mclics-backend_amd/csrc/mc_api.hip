@@ -330,6 +330,38 @@ int launch_sweep_graph(mc_handle *h, int n, hipGraphExec_t ge) {
   return MC_OK;
 }
 
+// n sweeps as graphs of at most kGraphChunk sweeps: the graph of kGraphChunk
+// sweeps replayed n / kGraphChunk times, then the graph of the n % kGraphChunk
+// that remain.  No graph holds more than kGraphChunk sweeps (8 * kGraphChunk
+// kernel nodes with the site kernel's eight passes), and a handle caches at
+// most 2 * kGraphChunk graphs (one per length and plane set).  kGraphChunk
+// is even, so every replay of the full chunk starts from the plane set the
+// remainder graph was captured on.
+constexpr int kGraphChunk = 64;
+static_assert(kGraphChunk % 2 == 0, "the chunk must keep the slab kernel's plane set");
+
+struct GraphPlan {
+  hipGraphExec_t chunk = nullptr, rest = nullptr;
+  int reps = 0, rem = 0;
+};
+
+int plan_sweep_graphs(mc_handle *h, int n, GraphPlan *g) {
+  g->reps = n / kGraphChunk;
+  g->rem = n % kGraphChunk;
+  int rc = MC_OK;
+  if (g->reps) rc = sweep_graph(h, kGraphChunk, &g->chunk);
+  if (rc == MC_OK && g->rem) rc = sweep_graph(h, g->rem, &g->rest);
+  return rc;
+}
+
+int launch_sweep_graphs(mc_handle *h, const GraphPlan &g) {
+  for (int r = 0; r < g.reps; ++r) {
+    const int rc = launch_sweep_graph(h, kGraphChunk, g.chunk);
+    if (rc) return rc;
+  }
+  return g.rem ? launch_sweep_graph(h, g.rem, g.rest) : MC_OK;
+}
+
 int ensure_parity_buffers(mc_handle *h) {
   const size_t nt = (size_t)h->P.nt;
   if (!h->pbuf) MC_HIP(hipMalloc(&h->pbuf, 7 * nt * sizeof(float)));
@@ -879,10 +911,10 @@ int mc_sweeps_graph(mc_handle *h, int n) {
   if (n < 0) return fail(MC_EINVAL, "mc_sweeps_graph: negative sweep count");
   if (n == 0) return MC_OK;
   if (h->learned) return mc_sweeps(h, n);  // the learned/plain pattern depends on the sweep count
-  hipGraphExec_t ge = nullptr;
-  rc = sweep_graph(h, n, &ge);
+  GraphPlan g;
+  rc = plan_sweep_graphs(h, n, &g);
   if (rc) return rc;
-  return launch_sweep_graph(h, n, ge);
+  return launch_sweep_graphs(h, g);
 }
 
 int mc_sync(mc_handle *h) {
@@ -897,16 +929,16 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   if (rc) return rc;
   if (n < 1 || !ms) return fail(MC_EINVAL, "mc_time_sweeps: need n >= 1 and an output");
   if (use_graph && h->learned) use_graph = 0;
-  hipGraphExec_t ge = nullptr;
+  GraphPlan g;
   if (use_graph) {
     // Instantiate before timing so capture cost stays outside the region.
     MC_HIP(hipStreamSynchronize(h->stream));
-    rc = sweep_graph(h, n, &ge);
+    rc = plan_sweep_graphs(h, n, &g);
     if (rc) return rc;
   }
   const int per = passes_per_sweep(h);
   MC_HIP(hipEventRecord(h->ev0, h->stream));
-  rc = use_graph ? launch_sweep_graph(h, n, ge) : mc_sweeps(h, n);
+  rc = use_graph ? launch_sweep_graphs(h, g) : mc_sweeps(h, n);
   if (rc) return rc;
   MC_HIP(hipEventRecord(h->ev1, h->stream));
   MC_HIP(hipEventSynchronize(h->ev1));

@@ -344,6 +344,25 @@ int launch_graph(mc_batch *b, int n, hipGraphExec_t ge) {
   return MC_OK;
 }
 
+// n sweeps as graphs of at most kGraphChunk sweeps (the full chunk replayed
+// n / kGraphChunk times, then the remainder), so no graph grows with n and
+// the cache holds at most 2 * kGraphChunk graphs.  The chunk is even, so the
+// remainder graph starts from the plane set it was captured on.
+constexpr int kGraphChunk = 64;
+static_assert(kGraphChunk % 2 == 0, "the chunk must keep the slab kernel's plane set");
+
+int run_graphs(mc_batch *b, int n, bool prepare_only) {
+  const int reps = n / kGraphChunk, rem = n % kGraphChunk;
+  hipGraphExec_t chunk = nullptr, rest = nullptr;
+  int rc = MC_OK;
+  if (reps) rc = get_graph(b, kGraphChunk, &chunk);
+  if (rc == MC_OK && rem) rc = get_graph(b, rem, &rest);
+  if (rc || prepare_only) return rc;
+  for (int r = 0; r < reps && rc == MC_OK; ++r) rc = launch_graph(b, kGraphChunk, chunk);
+  if (rc == MC_OK && rem) rc = launch_graph(b, rem, rest);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -512,10 +531,7 @@ int mc_batch_sweeps(mc_batch *b, int n) {
   if (rc) return rc;
   if (n < 0) return bfail(MC_EINVAL, "mc_batch_sweeps: negative sweep count");
   if (n == 0) return MC_OK;
-  hipGraphExec_t ge = nullptr;
-  rc = get_graph(b, n, &ge);
-  if (rc) return rc;
-  return launch_graph(b, n, ge);
+  return run_graphs(b, n, false);
 }
 
 int mc_batch_sync(mc_batch *b) {
@@ -529,11 +545,10 @@ int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
   int rc = check_batch(b);
   if (rc) return rc;
   if (n < 1 || !ms) return bfail(MC_EINVAL, "mc_batch_time_sweeps: need n >= 1 and an output");
-  hipGraphExec_t ge = nullptr;
-  rc = get_graph(b, n, &ge);
+  rc = run_graphs(b, n, true);  // capture and instantiate outside the timed region
   if (rc) return rc;
   MB_HIP(hipEventRecord(b->ev0, b->stream));
-  rc = launch_graph(b, n, ge);
+  rc = run_graphs(b, n, false);
   if (rc) return rc;
   MB_HIP(hipEventRecord(b->ev1, b->stream));
   MB_HIP(hipEventSynchronize(b->ev1));

@@ -270,6 +270,59 @@ def test_graph_replay_matches_eager():
             assert a.acceptance() == b.acceptance()
 
 
+@pytest.mark.parametrize("case", ["slab", "x4", "generic", "generic_ref8"])
+def test_chunked_graphs_match_eager(case):
+    # mc_sweeps_graph splits n sweeps into graphs of at most 64 sweeps (the
+    # full chunk replayed, then the remainder).  Odd counts above the chunk,
+    # an exact chunk and a single sweep, in a row, must leave the chain the
+    # eager launches give: directors, counters, ang_var and energy.
+    rng = np.random.default_rng(15)
+    dims = (8, 6, 6) if case == "generic_ref8" else (16, 12, 10)
+    nt = dims[0] * dims[1] * dims[2]
+    n = unit(rng, nt)
+    kind = {"slab": KERNEL_SLAB, "x4": KERNEL_X4, "generic": KERNEL_GENERIC, "generic_ref8": KERNEL_GENERIC}[case]
+    order = ORDER_REF8 if case == "generic_ref8" else ORDER_CHECKER
+    bound = (0, 1, 0) if case == "generic_ref8" else (1, 1, 1)
+    cfg = mclics.Config(*dims, bound=bound, T=1.05, seed=23, order=order)
+    with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
+        for lat in (a, b):
+            lat.set_kernel(kind)
+        for count in (165, 64, 1, 129):
+            a.sweeps(count)
+            b.sweeps(count, graph=True)
+            a.sync()
+            b.sync()
+            assert np.array_equal(a.directors(), b.directors()), (case, count)
+            assert a.acceptance() == b.acceptance()
+            assert a.ang_var() == b.ang_var()
+            assert a.measure()[0] == b.measure()[0]
+        ms, launches = b.time_sweeps(131, graph=True)
+        a.sweeps(131)
+        assert ms > 0 and np.array_equal(a.directors(), b.directors())
+
+
+def test_batch_chunked_graphs_keep_the_chain():
+    # A batch's sweeps run as graphs of at most 64 sweeps; 133 sweeps in one
+    # call equal 7 calls of 19 and, for each replica, the lattice run alone.
+    sizes = [(16, 16, 16), (8, 8, 8), (12, 6, 10)]
+    cfgs = [mclics.Config(*d, T=1.0 + 0.1 * r, seed=500 + r, stream=r) for r, d in enumerate(sizes)]
+    ics = [mclics.random_ic(*d, seed=500 + r) for r, d in enumerate(sizes)]
+    with mclics.Batch(cfgs, ics) as b1, mclics.Batch(cfgs, ics) as b2:
+        b1.sweeps(133)
+        for _ in range(7):
+            b2.sweeps(19)
+        for r in range(len(sizes)):
+            assert np.array_equal(b1.directors(r), b2.directors(r)), r
+        a1, s1, v1 = b1.state()
+        a2, s2, v2 = b2.state()
+        assert np.array_equal(a1, a2) and np.array_equal(s1, s2) and np.array_equal(v1, v2)
+        assert np.all(s1 == 133)
+        with mclics.Lattice(cfgs[2], ics[2]) as lat:
+            lat.set_kernel(KERNEL_X4)
+            lat.sweeps(133)
+            assert np.array_equal(lat.directors(), b1.directors(2))
+
+
 def test_full_size_256_properties():
     # The headline lattice: 256^3 LL, random start (seed 2).  Size-independent
     # properties: unit norms kept, counters, E inside [-9A, 0], S in [0, 1],
