@@ -32,12 +32,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """(threads, description) for the CPU legs: OMP_NUM_THREADS when it is
+    set (the GPU box sets it to the process's CPU share), else every core in
+    this process's affinity mask; with the host CPU model."""
+    avail = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = env if env > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"cpu_model": model, "affinity_cores": avail, "online_cpus": os.cpu_count(),
+                     "threads_from": "OMP_NUM_THREADS" if env > 0 else "sched_getaffinity"}
+
+
 def cpu_baseline(n0, budget_s=15.0):
     """The oracle's OpenMP restatement of the reference CPU sweep
     (src/evolve.cpp:64-179, 8 sublattice passes) on the host cores, timed on a
     bounded number of 256^3 sweeps at T=1.2."""
     import _oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, host = host_cores()
     p = O.params(L, L, L, A=1.0)
     n = n0.copy()
     pt = np.ones(L**3, np.int32)
@@ -58,7 +78,7 @@ def cpu_baseline(n0, budget_s=15.0):
     return {"value": rate, "unit": "site-updates/s", "cores": threads, "kind": "port",
             "sample": f"{reps} sweeps of the 256^3 LL lattice at T=1.2 (after 1 warm-up sweep), "
                       f"reference CPU order (8 sublattice passes), OpenMP static schedule, same Philox proposals",
-            "value_1core": one}
+            "value_1core": one, **host}
 
 
 def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40, schedule=None):
@@ -202,7 +222,7 @@ def c1_run(run_cpu, mct=500, mcs=200):
         # the first rows of the same run, as many as fit in about budget_s
         import _oracle as O
         budget_s = 12.0
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads, _ = host_cores()
         p = O.params(L1, L1, L1, A=1.0)
         n = n0.copy()
         pt = np.ones(L1**3, np.int32)

@@ -1,6 +1,7 @@
 // Evolution mode drivers of the host drop-in.
 #include "evolve.h"
 
+#include <ctype.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -56,11 +57,49 @@ inline bool resolve(int bound, int &ii, int N) {
 
 // Number of rows of the reference's float loops
 // for(x=xi; (int)1e6*sign*(x-xf)>=0; x+=dx) with sign = (int)(-dx/|dx|).
+// The body returns true to end the loop early.
 template <typename F>
 void float_loop(float xi, float xf, float dx, F body) {
   if (dx == 0) throw ConfigError{2, "the loop step (dT or dE) must not be zero"};
   const int sign = (int)(-dx / fabsf(dx));
-  for (float x = xi; (float)(1000000 * sign) * (x - xf) >= 0; x += dx) body(x);
+  for (float x = xi; (float)(1000000 * sign) * (x - xf) >= 0; x += dx)
+    if (body(x)) break;
+}
+
+// 64-bit FNV-1a.
+struct Fnv {
+  uint64_t h = 0xcbf29ce484222325ull;
+  void add(const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  }
+  void add_str(const char *s) { add(s, strlen(s) + 1); }
+  template <typename T>
+  void add_val(T v) {
+    add(&v, sizeof(v));
+  }
+};
+
+// Checkpoint file, little-endian:
+//   "MCLCKPT1", u32 version (1), i32 Nx, Ny, Nz, char evol[20],
+//   u64 fingerprint of the run's parameters and lattice, u32 index of the
+//   next sweep, f32 ang_var, i32 rows done, f32 loop variable of the next
+//   row, f32 directors[3*Nt] (AoS), then u64 FNV-1a of all bytes before it.
+constexpr char kMagic[8] = {'M', 'C', 'L', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr size_t kCkptHead = 8 + 4 + 12 + 20 + 8 + 4 + 4 + 4 + 4;
+
+template <typename T>
+void put(std::vector<unsigned char> &b, T v) {
+  const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+  b.insert(b.end(), p, p + sizeof(T));
+}
+template <typename T>
+T get(const unsigned char *&p) {
+  T v;
+  memcpy(&v, p, sizeof(T));
+  p += sizeof(T);
+  return v;
 }
 
 }  // namespace
@@ -167,6 +206,7 @@ void EvolveNGPU::snapshot(const std::string &fname) {
 FILE *EvolveNGPU::open_po(const char *first_col) {
   FILE *f = fopen("po.dat", "a");
   if (!f) throw std::runtime_error("cannot open po.dat");
+  if (rows_done > 0) return f;  // a resumed run appends to the rows already there
   if (strcasecmp(params->potential, "pear") == 0)
     fprintf(f, "%s S varS E varE P\n", first_col);
   else
@@ -193,15 +233,146 @@ double EvolveNGPU::polarization() const {
   return sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
 }
 
+// What the chain and the rows written so far depend on: mode, lattice,
+// boundaries, potential and its constants, field, anchoring, point types and
+// surface normals, seed, sweep counts and loop start and step.  The loop ends
+// (Tf, elecEf, fn) are left out, so a resumed run may go further or stop
+// earlier than the one that wrote the file.
+uint64_t EvolveNGPU::chain_fingerprint() const {
+  Fnv f;
+  const Parameters &p = *params;
+  char evol[20] = {0};
+  snprintf(evol, sizeof(evol), "%s", p.evol);
+  for (char &c : evol) c = (char)tolower((unsigned char)c);
+  f.add_str(evol);
+  f.add_str(geometry->name());
+  f.add_str(p.potential);
+  for (int v : {p.Nx, p.Ny, p.Nz, geometry->bound[0], geometry->bound[1], geometry->bound[2], p.MCT, p.MCS, p.seed,
+                p.first_file, cpu_semantics ? 1 : 0})
+    f.add_val(v);
+  for (float v : {p.A, p.B1, p.B2, p.C, p.ghrl_rho, p.ghrl_lambda, p.ghrl_mu, p.ghrl_nu, p.ghrl_sigma, p.elecX,
+                  p.elecY, p.elecZ, p.elecA, p.elecEi, p.elecdE, p.Ti, p.dT})
+    f.add_val(v);
+  for (const Anchoring &a : geometry->surfaces) {
+    f.add_val(a.kind);
+    f.add_val(a.W);
+    f.add(a.n_s, sizeof(a.n_s));
+  }
+  const size_t nt = (size_t)Nx * Ny * Nz;
+  f.add(pt, nt * sizeof(int));
+  if (!geometry->ns.empty()) f.add(geometry->ns.data(), geometry->ns.size() * sizeof(float));
+  return f.h;
+}
+
+void EvolveNGPU::start_run(float *x) {
+  rows_done = rows_this_run = 0;
+  const size_t nt = (size_t)Nx * Ny * Nz;
+  std::vector<unsigned char> b;
+  if (!control.resume.empty()) {
+    // Read and check the whole file before the device is touched.
+    FILE *f = fopen(control.resume.c_str(), "rb");
+    if (!f) throw ConfigError{2, "cannot open checkpoint " + control.resume};
+    unsigned char chunk[1 << 16];
+    size_t got;
+    while ((got = fread(chunk, 1, sizeof(chunk), f)) > 0) b.insert(b.end(), chunk, chunk + got);
+    fclose(f);
+    const std::string what = "checkpoint " + control.resume + ": ";
+    if (b.size() != kCkptHead + 12 * nt + 8)
+      throw ConfigError{2, what + "its size does not fit a " + std::to_string(Nx) + "x" + std::to_string(Ny) + "x" +
+                               std::to_string(Nz) + " lattice"};
+    Fnv sum;
+    sum.add(b.data(), b.size() - 8);
+    uint64_t stored;
+    memcpy(&stored, b.data() + b.size() - 8, 8);
+    if (memcmp(b.data(), kMagic, 8) != 0) throw ConfigError{2, what + "not a checkpoint file"};
+    if (stored != sum.h) throw ConfigError{2, what + "checksum mismatch (file damaged)"};
+    const unsigned char *q = b.data() + 8;
+    if (get<uint32_t>(q) != kCkptVersion) throw ConfigError{2, what + "unknown version"};
+    const int nx = get<int32_t>(q), ny = get<int32_t>(q), nz = get<int32_t>(q);
+    q += 20;  // evol: covered by the fingerprint
+    const uint64_t fp = get<uint64_t>(q);
+    if (nx != Nx || ny != Ny || nz != Nz || fp != chain_fingerprint())
+      throw ConfigError{2, what + "written by a run with other parameters, lattice or mode"};
+  }
+  open();
+  if (b.empty()) return;
+  const unsigned char *q = b.data() + 8 + 4 + 12 + 20 + 8;
+  const uint32_t sweep = get<uint32_t>(q);
+  const float av = get<float>(q);
+  const int rows = get<int32_t>(q);
+  const float nx = get<float>(q);
+  memcpy(ni, q, 12 * nt);
+  check(mc_set_directors_aos(h, ni), "mc_set_directors_aos");
+  check(mc_set_ang_var(h, av), "mc_set_ang_var");
+  check(mc_set_sweep_index(h, sweep), "mc_set_sweep_index");
+  rows_done = rows;
+  *x = nx;
+  printf("Resuming from %s: %d row(s) done, next sweep %u, ang_var %g\n", control.resume.c_str(), rows, sweep,
+         (double)av);
+  fflush(stdout);
+}
+
+void EvolveNGPU::write_checkpoint(float next_x) {
+  const size_t nt = (size_t)Nx * Ny * Nz;
+  fetch();
+  float av = 0;
+  uint32_t acc = 0, sweep = 0;
+  check(mc_get_ang_var(h, &av), "mc_get_ang_var");
+  check(mc_get_acceptance(h, &acc, &sweep), "mc_get_acceptance");
+  std::vector<unsigned char> b;
+  b.reserve(kCkptHead + 12 * nt + 8);
+  b.insert(b.end(), kMagic, kMagic + 8);
+  put<uint32_t>(b, kCkptVersion);
+  put<int32_t>(b, Nx);
+  put<int32_t>(b, Ny);
+  put<int32_t>(b, Nz);
+  char evol[20] = {0};
+  snprintf(evol, sizeof(evol), "%s", params->evol);
+  b.insert(b.end(), evol, evol + 20);
+  put<uint64_t>(b, chain_fingerprint());
+  put<uint32_t>(b, sweep);
+  put<float>(b, av);
+  put<int32_t>(b, rows_done);
+  put<float>(b, next_x);
+  const unsigned char *d = reinterpret_cast<const unsigned char *>(ni);
+  b.insert(b.end(), d, d + 12 * nt);
+  Fnv sum;
+  sum.add(b.data(), b.size());
+  put<uint64_t>(b, sum.h);
+  // Written beside the target and renamed over it, so an interrupted write
+  // leaves the previous checkpoint whole.
+  const std::string tmp = control.checkpoint + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write checkpoint " + tmp);
+  const bool ok = fwrite(b.data(), 1, b.size(), f) == b.size();
+  if (fclose(f) != 0 || !ok) throw std::runtime_error("short write of checkpoint " + tmp);
+  if (rename(tmp.c_str(), control.checkpoint.c_str()) != 0)
+    throw std::runtime_error("cannot rename " + tmp + " to " + control.checkpoint);
+}
+
+bool EvolveNGPU::after_row(float next_x) {
+  ++rows_done;
+  ++rows_this_run;
+  if (!control.checkpoint.empty()) write_checkpoint(next_x);
+  if (control.max_rows > 0 && rows_this_run >= control.max_rows) {
+    printf("Stopping after %d row(s) of this run (%d in all)%s%s\n", rows_this_run, rows_done,
+           control.checkpoint.empty() ? "" : "; checkpoint ", control.checkpoint.c_str());
+    fflush(stdout);
+    return true;
+  }
+  return false;
+}
+
 // src/evolve_thermal.cu:33-116 (CPU twin src/evolve_thermal.cpp:30-95).
 int thermalEvolveNGPU::run() {
   params->T = params->Ti;
-  open();
+  float T0 = params->Ti;
+  start_run(&T0);
   FILE *po = open_po("T");
   printf("Starting thermal variation from %g to %g with step %g\nMCT=%d MCS=%d\n", params->Ti, params->Tf,
          params->dT, params->MCT, params->MCS);
   fflush(stdout);
-  float_loop(params->Ti, params->Tf, params->dT, [&](float T) {
+  float_loop(T0, params->Tf, params->dT, [&](float T) {
     set_T(T);
     sweeps(params->MCT);
     double E, E2, S1, S2;
@@ -212,6 +383,7 @@ int thermalEvolveNGPU::run() {
     snapshot(name);
     snprintf(x, sizeof(x), "%g", (double)T);
     row(po, x, E, E2, S1, S2);
+    return after_row(T + params->dT);
   });
   fclose(po);
   return 0;
@@ -220,16 +392,18 @@ int thermalEvolveNGPU::run() {
 // src/evolve_step.cu:31-115.
 int stepEvolveNGPU::run() {
   params->T = params->Ti;
-  open();
+  float unused = 0;
+  start_run(&unused);
   FILE *po = open_po("ii");
   printf("Step relaxation using MCT=%d MCS=%d and fn=%d\n", params->MCT, params->MCS, params->fn);
   fflush(stdout);
-  for (int ii = params->first_file; ii < params->fn + params->first_file; ++ii) {
+  for (int ii = params->first_file + rows_done; ii < params->fn + params->first_file; ++ii) {
     sweeps(params->MCT);
     double E, E2, S1, S2;
     measured(params->MCS, E, E2, S1, S2);
     snapshot("director_field_" + std::to_string(ii) + ".csv");
     row(po, std::to_string(ii).c_str(), E, E2, S1, S2);
+    if (after_row(0.0f)) break;
   }
   fclose(po);
   return 0;
@@ -239,7 +413,8 @@ int stepEvolveNGPU::run() {
 // (the CPU build runs every integer below MCT*dT, src/evolve_quench.cpp:62).
 int quenchEvolveNGPU::run() {
   params->T = params->Ti;
-  open();
+  float unused = 0;
+  start_run(&unused);
   FILE *po = open_po("ii");
   const float prod = params->MCT * params->dT;
   int second = cpu_semantics ? (prod > 0 ? (int)ceilf(prod) : 0) : (int)prod;
@@ -247,7 +422,7 @@ int quenchEvolveNGPU::run() {
   printf("Quench: MCT=%d at T=%g, %d sweeps and MCS=%d at T=%g, fn=%d\n", params->MCT, params->Ti, second,
          params->MCS, params->Tf, params->fn);
   fflush(stdout);
-  for (int ii = params->first_file; ii < params->fn + params->first_file; ++ii) {
+  for (int ii = params->first_file + rows_done; ii < params->fn + params->first_file; ++ii) {
     set_T(params->Ti);
     sweeps(params->MCT);
     set_T(params->Tf);
@@ -256,6 +431,7 @@ int quenchEvolveNGPU::run() {
     measured(params->MCS, E, E2, S1, S2);
     snapshot("director_field_" + std::to_string(ii) + ".csv");
     row(po, std::to_string(ii).c_str(), E, E2, S1, S2);
+    if (after_row(0.0f)) break;
   }
   fclose(po);
   return 0;
@@ -265,12 +441,13 @@ int quenchEvolveNGPU::run() {
 int electricEvolveNGPU::run() {
   params->T = params->Ti;
   params->elecE = params->elecEi;
-  open();
+  float E0 = params->elecEi;
+  start_run(&E0);
   FILE *po = open_po("T");
   printf("Starting electric variation from %g to %g with step %g\nMCT=%d MCS=%d\n", params->elecEi,
          params->elecEf, params->elecdE, params->MCT, params->MCS);
   fflush(stdout);
-  float_loop(params->elecEi, params->elecEf, params->elecdE, [&](float E) {
+  float_loop(E0, params->elecEf, params->elecdE, [&](float E) {
     set_field(E);
     sweeps(params->MCT);
     double Em, E2, S1, S2;
@@ -281,6 +458,7 @@ int electricEvolveNGPU::run() {
     snapshot(name);
     snprintf(x, sizeof(x), "%g", (double)E);
     row(po, x, Em, E2, S1, S2);
+    return after_row(E + params->elecdE);
   });
   fclose(po);
   return 0;

@@ -7,6 +7,8 @@
 // stays on the GPU for the whole run and only E and S per measurement sweep
 // (FP64, reduced on the device) and one snapshot per outer step come back.
 #pragma once
+#include <stdint.h>
+
 #include <string>
 
 #include "../../include/mclics_hip.h"
@@ -15,12 +17,23 @@
 
 namespace mc {
 
+// Checkpointing of a run (no reference counterpart: the reference restarts
+// from a director snapshot only, src/ic.cpp:88-119).  With `checkpoint` set,
+// run() writes the exact state of the chain after every outer row (po.dat
+// row); with `resume` set it continues the run such a file was written by;
+// max_rows > 0 ends this invocation after that many rows.
+struct RunControl {
+  std::string checkpoint, resume;
+  int max_rows = 0;
+};
+
 class Evolve {
  public:
   virtual ~Evolve() = default;
   virtual int run() { return 0; }
   Geometry *geometry = nullptr;
   int VallidPoints = 0;
+  RunControl control;
   // src/evolve.cpp:180-193
   void check_Points(int *pt, const Parameters &params);
 };
@@ -47,6 +60,18 @@ class EvolveNGPU : public Evolve {
   FILE *open_po(const char *first_col);
   void row(FILE *f, const char *x, double E, double E2, double S1, double S2);
   double polarization() const;
+
+  // Checkpoints.  start_run() checks control.resume against this run before
+  // any device work, opens the handle, restores the chain from the file and
+  // sets rows_done and *x (the loop variable of the next row); without a
+  // resume file it only opens the handle.  after_row() counts the row,
+  // writes control.checkpoint and says whether this invocation stops here.
+  void start_run(float *x);
+  bool after_row(float next_x);
+  void write_checkpoint(float next_x);
+  uint64_t chain_fingerprint() const;
+  int rows_done = 0;
+  int rows_this_run = 0;
 
   float *ni;
   int *pt;

@@ -17,7 +17,7 @@ struct Anchoring {
   std::string type;   // parameter-file name, e.g. "homeotropic_ghrl"
   int kind = 0;       // mc_anchoring
   float W = 0;
-  float n_s[3] = {0, 0, 1};  // (theta_s, phi_s) direction for rp/strong kinds
+  float n_s[3] = {0, 0, 1};  // unit vector of (theta_s, phi_s) for rp/strong kinds
   const char *name() const;  // the reference's class name string
 };
 

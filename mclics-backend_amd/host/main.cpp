@@ -6,8 +6,17 @@
 // errors.  `mc_sim --dry-run <paramfile>` stops after ic.csv, writes the
 // initial field as director_field_dryrun.csv and prints the resolved
 // configuration as one JSON line, without touching the GPU.
+//
+// Checkpoints (no reference counterpart):
+//   --checkpoint FILE  write the exact chain state to FILE after every po.dat row;
+//   --resume FILE      continue the run FILE was written by: the previous
+//                      outputs are kept, ic.csv is not rewritten and po.dat
+//                      gets the remaining rows, identical to those of an
+//                      uninterrupted run;
+//   --max-rows N       stop after N rows of this invocation.
 #include <dirent.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <stdexcept>
@@ -63,21 +72,39 @@ void dry_run_report(const mc::simulator &sim, const Parameters &p) {
 int main(int argc, char **argv) {
   bool dry = false;
   const char *file = nullptr;
+  mc::RunControl control;
   for (int a = 1; a < argc; ++a) {
-    if (!strcmp(argv[a], "--dry-run"))
+    const bool has_value = a + 1 < argc;
+    if (!strcmp(argv[a], "--dry-run")) {
       dry = true;
-    else
+    } else if (!strcmp(argv[a], "--checkpoint") && has_value) {
+      control.checkpoint = argv[++a];
+    } else if (!strcmp(argv[a], "--resume") && has_value) {
+      control.resume = argv[++a];
+    } else if (!strcmp(argv[a], "--max-rows") && has_value) {
+      control.max_rows = atoi(argv[++a]);
+      if (control.max_rows < 1) {
+        fprintf(stderr, "  [ERROR] --max-rows needs a count >= 1\n");
+        return 2;
+      }
+    } else if (argv[a][0] == '-' && argv[a][1] == '-') {
+      fprintf(stderr, "  [ERROR] unknown or incomplete option %s\n", argv[a]);
+      return 2;
+    } else {
       file = argv[a];
+    }
   }
   mc::print_header("McLiCS on MI355X - Monte Carlo Liquid Crystal Simulator", 60, "=");
   printf("\n");
   try {
-    clean_previous_output();
+    const bool resuming = !control.resume.empty() && !dry;
+    if (!resuming) clean_previous_output();
     Parameters params = mc::read_input_file(file);
     mc::print_parameters(params);
     mc::simulator sim(&params);
     sim.Setup_simmulation(params);
-    sim.print_n("ic.csv", &params);
+    if (!resuming) sim.print_n("ic.csv", &params);
+    sim.evolve->control = control;
     if (dry) {
       // The snapshot writer on the initial field, so it can be checked
       // without a device.

@@ -231,3 +231,67 @@ def test_custom_geometry_file(tmp_path):
     assert rep["n_surf"] == 2 and rep["valid"] == 11
     rc, rep, err = dry(tmp_path, "nx 2\nny 2\nnz 2\ngeometry custom\nboundary_file b.csv\n")
     assert rc == 10  # point out of the box
+
+
+# --- checkpoints ------------------------------------------------------------
+# File layout (little-endian): "MCLCKPT1", u32 version 1, i32 Nx Ny Nz,
+# char evol[20], u64 fingerprint, u32 next sweep, f32 ang_var, i32 rows done,
+# f32 next loop value, f32 directors[3*Nt], u64 FNV-1a of all preceding bytes.
+CKPT_PARAMS = "nx 4\nny 4\nnz 4\nevol thermalGPU\nTi 1.0\nTf 0.8\ndT -0.1\nMCT 2\nMCS 2\nseed 4\n"
+
+
+def fnv1a(b):
+    h = 0xcbf29ce484222325
+    for c in b:
+        h = ((h ^ c) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def ckpt_bytes(nt, magic=b"MCLCKPT1", fingerprint=0, good_sum=True):
+    import struct
+    body = magic + struct.pack("<I3i", 1, 4, 4, 4) + b"thermalGPU".ljust(20, b"\0")
+    body += struct.pack("<QIfif", fingerprint, 40, 0.5, 1, 0.9) + np.tile(np.float32([0, 0, 1]), nt).tobytes()
+    s = fnv1a(body) ^ (0 if good_sum else 1)
+    return body + struct.pack("<Q", s)
+
+
+def run_sim(tmp_path, *args):
+    _ensure_built()
+    (tmp_path / "p.txt").write_text(CKPT_PARAMS)
+    r = subprocess.run([SIM, *args, "p.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("case,needle", [
+    ("missing", "cannot open checkpoint"),
+    ("short", "does not fit a 4x4x4 lattice"),
+    ("magic", "not a checkpoint file"),
+    ("checksum", "checksum mismatch"),
+    ("fingerprint", "other parameters"),
+])
+def test_resume_rejects_bad_checkpoints_before_the_device(tmp_path, case, needle):
+    # Every check runs on the host before the device is opened, so each of
+    # these ends with the configuration status 2 even without a GPU.
+    f = tmp_path / "ck.bin"
+    if case == "short":
+        f.write_bytes(ckpt_bytes(64)[:-4])
+    elif case == "magic":
+        f.write_bytes(ckpt_bytes(64, magic=b"NOTACKPT"))
+    elif case == "checksum":
+        f.write_bytes(ckpt_bytes(64, good_sum=False))
+    elif case == "fingerprint":
+        f.write_bytes(ckpt_bytes(64, fingerprint=12345))
+    (tmp_path / "po.dat").write_text("T S varS E varE\nkept\n")
+    rc, out = run_sim(tmp_path, "--resume", "ck.bin")
+    assert rc == 2, out
+    assert needle in out
+    # a resumed run keeps the previous outputs
+    assert (tmp_path / "po.dat").read_text() == "T S varS E varE\nkept\n"
+
+
+@pytest.mark.parametrize("args", [["--max-rows", "0", "p.txt"], ["--bogus", "p.txt"], ["p.txt", "--checkpoint"]])
+def test_bad_checkpoint_options(tmp_path, args):
+    _ensure_built()
+    (tmp_path / "p.txt").write_text(CKPT_PARAMS)
+    r = subprocess.run([SIM, *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2, r.stdout + r.stderr
