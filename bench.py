@@ -246,19 +246,22 @@ def c1_run(run_cpu, mct=500, mcs=200):
     return out
 
 
-def slab_geometry_leg(n_sweeps=400):
-    """Config C3: 128x128x64 GHRL (k11, k22, k33 = 1.0, 0.7, 1.3; p0 = 0)
-    between homeotropic GHRL anchoring faces (W = 5), field along z (elecA = 1,
-    elecE = 0.5), T = 0.5, periodic x/y and free z, homogeneous start along x.
-    Sweeps are timed after autotuning the fast kernels; 25 B per site-update
-    (the 24 B of the directors plus the 1 B point type of the slab layout)."""
+def slab_geometry_leg(n_sweeps=400, p0=0.0):
+    """Config C3: 128x128x64 GHRL (k11, k22, k33 = 1.0, 0.7, 1.3; p0 = 0, or
+    p0 = 16 for the cholesteric variant) between homeotropic GHRL anchoring
+    faces (W = 5), field along z (elecA = 1, elecE = 0.5), T = 0.5, periodic
+    x/y and free z, homogeneous start along x.  Sweeps are timed after
+    autotuning the fast kernels; 25 B per site-update (the 24 B of the
+    directors plus the 1 B point type of the slab layout)."""
     import mclics
     from mclics.ic import homogeneous_ic
     nx, ny, nz = 128, 128, 64
     nt = nx * ny * nz
     k11, k22, k33 = 1.0, 0.7, 1.3
-    S = abs(9.0 / (k11 - 3 * k22 - k33))  # src/io.cpp:282-288, p0 = 0
-    ghrl = (S / 9 * (k11 - k33), S / 9 * (2 * k11 - 3 * k22 + k33), S * (k22 - k11), S / 9 * (k11 - 3 * k22 - k33), 0.0)
+    S = abs(9.0 / (k11 - 3 * k22 - k33))  # src/io.cpp:282-288
+    sigma = -S * k22 * (2 * np.pi / p0) if p0 else 0.0
+    ghrl = (S / 9 * (k11 - k33), S / 9 * (2 * k11 - 3 * k22 + k33), S * (k22 - k11), S / 9 * (k11 - 3 * k22 - k33),
+            sigma)
     pt = np.ones(nt, np.int32)
     ns = np.zeros(3 * nt, np.float32)
     pt[: nx * ny] = 2
@@ -276,8 +279,8 @@ def slab_geometry_leg(n_sweeps=400):
         kernel = {mclics.KERNEL_SLAB: "k_sweep_slab", mclics.KERNEL_X4: "k_sweep_x4"}.get(lat.kernel(), "k_sweep_pass")
         E, S_, _ = lat.measure()
     rate = n_sweeps * nt / (ms / 1e3)
-    return {"workload": "GHRL 128x128x64, periodic x/y, free z, homeotropic_ghrl faces W=5, field z elecA=1 "
-                        "elecE=0.5, T=0.5", "sites": nt, "sweeps": n_sweeps, "us_per_sweep": ms * 1e3 / n_sweeps,
+    return {"workload": f"GHRL 128x128x64 p0={p0:g}, periodic x/y, free z, homeotropic_ghrl faces W=5, field z "
+                        "elecA=1 elecE=0.5, T=0.5", "sites": nt, "sweeps": n_sweeps, "us_per_sweep": ms * 1e3 / n_sweeps,
             "kernel": kernel, "site_updates_per_s": rate,
             "roofline_frac_25B": rate * 25 / (HBM_PEAK_GBS * 1e9), "E": E, "S": S_}
 
@@ -437,6 +440,8 @@ def main():
     if not args.no_slab_geometry:
         log("bench: slab-geometry leg (GHRL 128x128x64, anchoring faces, field)")
         result["slab_geometry"] = slab_geometry_leg()
+        log("bench: slab-geometry leg, cholesteric variant (p0 = 16)")
+        result["slab_geometry_cholesteric"] = slab_geometry_leg(p0=16.0)
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
         result["learned_proposal"] = learned_leg(args.learned_sweeps, epochs=args.learned_epochs)

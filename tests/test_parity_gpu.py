@@ -43,6 +43,12 @@ def make_case(name, rng):
                                                               "elecE": 0.7}
     elif name == "pear_free":
         dims, bound, pot, kw = (5, 7, 4), (0, 0, 0), O.PEAR, {"A": 1.0, "B1": 0.3, "Cc": 0.3}
+    elif name == "slab_chiral":
+        # the cholesteric slab workload at a small size: GHRL with p0 = 16
+        # (sigma from src/io.cpp:284), homeotropic GHRL faces, field along z
+        dims, bound, pot = (8, 6, 6), (1, 1, 0), O.GHRL
+        surf = [("homeotropic_ghrl", 5.0, (0.0, 0.0, 1.0))] * 2
+        kw = {"k": GHRL_K, "elec": (0.0, 0.0, 1.0), "elecA": 1.0, "elecE": 0.5}
     elif name.startswith("slab_"):
         kind = name[5:]
         dims, bound, pot = (8, 6, 6), (1, 1, 0), O.GHRL
@@ -65,7 +71,7 @@ def make_case(name, rng):
     po = O.params(nx, ny, nz, bound=bound, potential=pot, surfaces=surf, **kw)
     pt = np.ones(nt, np.int32)
     ns = None
-    if name.startswith("slab_") or name == "ll_slab":
+    if name.startswith("slab_") or name == "ll_slab":  # slab_chiral included
         ns = np.zeros(3 * nt, np.float32)
         for j in range(ny):
             for i in range(nx):
@@ -280,8 +286,9 @@ def test_chunked_graphs_match_eager(case):
     dims = (8, 6, 6) if case == "generic_ref8" else (16, 12, 10)
     nt = dims[0] * dims[1] * dims[2]
     n = unit(rng, nt)
-    kind = {"slab": KERNEL_SLAB, "x4": KERNEL_X4, "generic": KERNEL_GENERIC, "generic_ref8": KERNEL_GENERIC}[case]
-    order = ORDER_REF8 if case == "generic_ref8" else ORDER_CHECKER
+    kind = {"slab": mclics.KERNEL_SLAB, "x4": mclics.KERNEL_X4, "generic": mclics.KERNEL_GENERIC,
+            "generic_ref8": mclics.KERNEL_GENERIC}[case]
+    order = mclics.ORDER_REF8 if case == "generic_ref8" else mclics.ORDER_CHECKER
     bound = (0, 1, 0) if case == "generic_ref8" else (1, 1, 1)
     cfg = mclics.Config(*dims, bound=bound, T=1.05, seed=23, order=order)
     with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
@@ -318,7 +325,7 @@ def test_batch_chunked_graphs_keep_the_chain():
         assert np.array_equal(a1, a2) and np.array_equal(s1, s2) and np.array_equal(v1, v2)
         assert np.all(s1 == 133)
         with mclics.Lattice(cfgs[2], ics[2]) as lat:
-            lat.set_kernel(KERNEL_X4)
+            lat.set_kernel(mclics.KERNEL_X4)
             lat.sweeps(133)
             assert np.array_equal(lat.directors(), b1.directors(2))
 
@@ -510,7 +517,7 @@ def test_segment_kernel_pipelined_grid_keeps_the_chain(dims, pot, zfree, faces, 
 
 
 SLAB_GEOMETRY = ["slab_" + k for k in ("homeotropic", "rp", "fg", "strong", "homeotropic_ghrl", "rp_ghrl",
-                                        "fg_ghrl", "strong_ghrl")] + ["ll_slab", "ll_freez"]
+                                        "fg_ghrl", "strong_ghrl", "chiral")] + ["ll_slab", "ll_freez"]
 
 
 @pytest.mark.parametrize("kernel", ["x4", "slab"])
