@@ -15,8 +15,10 @@
 // halo) reaches the same proposal and decision.
 //
 // Accept decision.  The energy difference is first formed in FP32: for
-// Lebwohl-Lasher with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2); for GHRL and
-// pear bond by bond with the reference's FP64 steps in FP32.  Its distance
+// Lebwohl-Lasher with FMAs, sum_b c*((q.m_b)^2 - (o.m_b)^2); for GHRL with
+// FMAs from a form of the bond whose constant terms cancel between the two
+// directors; for pear bond by bond with the reference's FP64 steps in FP32.
+// Its distance
 // from the reference's own FP32/FP64 arithmetic has a proven bound (below),
 // and the band eps around exp(-dE/T) keeps a several-fold margin over it.  A
 // hardware exp2 decides every move whose uniform lies outside the band; moves
@@ -25,6 +27,7 @@
 // Surface sites are always decided exactly.
 #pragma once
 #include "mc_kernels.h"
+#include "mc_ghrl_screen.h"
 
 namespace mcsu {
 
@@ -140,6 +143,7 @@ struct PassConst {
   float cAf, eps;
   float ky1, kyLL;  // -RN(1/T) log2(e) and that times cAf: exp2 arguments of dE and of the LL bond sum
   float relc;       // RN(eps/T) + 2^-19 for the FP32 screen
+  McGhrlConst ghrl;  // GHRL screen constants
   float hs, hb;     // ang_var 2^-24 and -0.5 ang_var (propose)
 };
 
@@ -165,9 +169,10 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
   const float e2 = P.eX * P.eX + P.eY * P.eY + P.eZ * P.eZ;
   const float F = fabsf(P.elecA) * k.elecE * k.elecE * e2;
   // 2^-14 * (...): LL with the bound 1.5|A| itself (screen_ll); GHRL and pear
-  // with twice the bond bound (screen_bonds)
+  // with twice the bond bound (screen_ghrl, screen_bonds)
   const float M = bond_bound(P);
   k.eps = 6.1035156e-5f * ((P.potential == MCD_LL ? M : 2.0f * M) + F);
+  k.ghrl = mcx_ghrl_const(P.rho, P.lambda, P.mu, P.nu, P.sigma);
   k.ky1 = -((1.0f / k.T) * 1.44269504088896341f);
   k.kyLL = k.cAf * k.ky1;
   k.relc = k.eps / k.T + 1.9073486e-6f;
@@ -206,39 +211,22 @@ __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &
   return fast_decide(y, S.u, k.relc);
 }
 
-// FP32 screen of one GHRL or pear move: each bond of the reference
-// (src/potential.cpp:10-38) with every FP64 step done in FP32.  n.r, m.r
-// and (n x m).r are read off as signed components (r is a lattice axis), the
-// rest is the reference's FP32 arithmetic.  Error (u = 2^-24, unit vectors,
-// M = bond_bound): the reference's n_ij, P2, cross, nu P2 and sigma sg cross
-// are the same FP32 operations here, so the sign of n_ij agrees and the
-// chiral term's jump cannot differ; E1 in FP32 is off by <= 8u, so E1 (rho P2
-// + lambda) by <= 9u (|rho| + |lambda|); mu (a a n - 1/9) by <= 2.3u |mu|;
-// the three FP32 additions of a bond by <= 2u M; pear's f - 0.25 C ps by
-// <= 3u M.  So each bond is within 12u M of the reference's (FP64 sum rounded
-// once), twelve of them within 144u M, the FP32 accumulation of the twelve
-// differences adds <= 48u M, the reference's own roundings of its two site
-// energies and their difference <= 12u M, the field terms (the same FP32
-// operations) <= 4u F: <= 210u M + 4u F against eps = 2^-14 (2M + F) =
-// 2048u M + 1024u F, a tenfold margin.
-template <int POT>
-__device__ __forceinline__ float bond32(const MCDevParams &P, int b, const float n[3], const float m[3]) {
+// FP32 screen of one pear move: each bond of the reference
+// (src/potential.cpp:10-15) with every FP64 step done in FP32.  n.r and m.r
+// are read off as signed components (r is a lattice axis), the rest is the
+// reference's FP32 arithmetic.  Error (u = 2^-24, unit vectors, M =
+// bond_bound): the reference's n_ij and the products are the same FP32
+// operations here; f - 0.25 C ps is off by <= 3u M and the FP32 additions of
+// a bond by <= 2u M.  So each bond is within 12u M of the reference's (FP64
+// sum rounded once), twelve of them within 144u M, the FP32 accumulation of
+// the twelve differences adds <= 48u M, the reference's own roundings of its
+// two site energies and their difference <= 36u M, the field terms (the same
+// FP32 operations) <= 4u F: <= 230u M + 4u F against eps = 2^-14 (2M + F) =
+// 2048u M + 1024u F, an eightfold margin.
+__device__ __forceinline__ float bond32_pear(const MCDevParams &P, int b, const float n[3], const float m[3]) {
   const int a = b >> 1;
   const float sg = (b & 1) ? -1.0f : 1.0f;
   const float nij = mcd_dot(n[0], n[1], n[2], m[0], m[1], m[2]);
-  if (POT == MCD_GHRL) {
-    const float ai = sg * n[a], aj = sg * m[a];
-    const float pij = 1.5f * nij * nij - 0.5f;
-    const float ca = a == 0 ? n[2] * m[1] - n[1] * m[2] : (a == 1 ? n[0] * m[2] - n[2] * m[0] : n[1] * m[0] - n[0] * m[1]);
-    const float cross = sg * ca;
-    float E1 = 1.5f * ai * ai;
-    E1 = E1 + 1.5f * aj * aj;
-    E1 = E1 - 1.0f;
-    const float t1 = E1 * (P.rho * pij + P.lambda);
-    const float t2 = P.mu * (ai * aj * nij - 0.111111112f);
-    const float ss = nij > 0 ? 1.0f : -1.0f;
-    return ((t1 + t2) + P.nu * pij) + P.sigma * ss * cross;
-  }
   float aa = 1.0f + 2.0f * nij;
   aa = aa + nij * nij;
   const float d = sg * (m[a] - n[a]);
@@ -248,16 +236,30 @@ __device__ __forceinline__ float bond32(const MCDevParams &P, int b, const float
   return f - (P.C * 0.25f) * ps;
 }
 
-template <int POT, bool GEO>
+template <bool GEO>
 __device__ __forceinline__ int screen_bonds(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
   if (GEO && w.surf >= 0) return -1;
   float acc = 0.0f;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
     if (GEO && !((w.on >> b) & 1u)) continue;
-    acc += bond32<POT>(P, b, S.q, S.m[b]) - bond32<POT>(P, b, S.o, S.m[b]);
+    acc += bond32_pear(P, b, S.q, S.m[b]) - bond32_pear(P, b, S.o, S.m[b]);
   }
   float dE = acc;
+  if (P.elecA != 0.0f)
+    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
+  return fast_decide(dE * k.ky1, S.u, k.relc);
+}
+
+// FP32 screen of one GHRL move: mcx_ghrl_delta (mc_ghrl_screen.h), whose
+// distance from the reference's dE is at most 920u M (+ 6u F with the
+// field) against eps = 2^-14 (2M + F) = 2048u M + 1024u F, over twice that.
+template <bool GEO>
+__device__ __forceinline__ int screen_ghrl(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
+  if (GEO && w.surf >= 0) return -1;
+  bool tiny;
+  float dE = mcx_ghrl_delta(k.ghrl, S.o, S.q, S.m, GEO ? w.on : 0x3Fu, &tiny);
+  if (tiny) return -1;
   if (P.elecA != 0.0f)
     dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
   return fast_decide(dE * k.ky1, S.u, k.relc);
@@ -306,9 +308,12 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   if (POT == MCD_LL) {
     d0 = screen_ll<GEO>(P, k, S[0], w);
     d1 = screen_ll<GEO>(P, k, S[1], w);
+  } else if (POT == MCD_GHRL) {
+    d0 = screen_ghrl<GEO>(P, k, S[0], w);
+    d1 = screen_ghrl<GEO>(P, k, S[1], w);
   } else {
-    d0 = screen_bonds<POT, GEO>(P, k, S[0], w);
-    d1 = screen_bonds<POT, GEO>(P, k, S[1], w);
+    d0 = screen_bonds<GEO>(P, k, S[0], w);
+    d1 = screen_bonds<GEO>(P, k, S[1], w);
   }
   a[0] = d0 == 1;
   a[1] = d1 == 1;

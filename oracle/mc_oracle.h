@@ -52,7 +52,7 @@ typedef struct {
   int32_t n_surf;
   int32_t surf_kind[ORC_MAX_SURF];
   float surf_W[ORC_MAX_SURF];
-  float surf_dir[ORC_MAX_SURF][3]; /* (theta_s, phi_s) direction for rp/strong kinds */
+  float surf_dir[ORC_MAX_SURF][3]; /* unit vector of (theta_s, phi_s) for rp/strong kinds */
 } orc_params;
 
 /* Pair, surface and field terms (src/potential.cpp, src/anchoring_*.cpp). */
