@@ -81,17 +81,61 @@ def cpu_baseline(n0, budget_s=15.0):
             "value_1core": one, **host}
 
 
-def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40, schedule=None):
+def chain_stats(lat, sweep_us, budget_s, target=0.2, min_sweeps=20000):
+    """E and S of a running chain, measured until tau_int(S) is resolved to
+    `target` relative error (and the chain holds min_sweeps sweeps) or the
+    budget in seconds is spent.  A first 20,000-sweep series with every sweep
+    measured sets the thinning k ~ tau/10 (1..200); tau in sweeps is then
+    k tau_int of the thinned series.  Returns a dict of the chain's figures."""
+    from mclics.decorrelation import tau_error, tau_int
+    t0 = time.perf_counter()
+    E0, S0 = lat.measure_series(20000)
+    tau0, _ = tau_int(S0)
+    every = int(min(200, max(1, tau0 // 10)))
+    Es, Ss = ([E0], [S0]) if every == 1 else ([], [])
+    # samples per call: about two seconds of sweeps (measurement ~16 us)
+    per = max(500, int(2.0e6 / (every * sweep_us + 16.0)))
+    last = t0
+    while True:
+        if Ss:
+            S = np.concatenate(Ss)
+            tau, M = tau_int(S)
+            err = tau_error(tau, M, len(S))
+            done = err <= target * tau and len(S) * every >= min_sweeps
+            now = time.perf_counter()
+            if done or now - t0 > budget_s:
+                break
+            if now - last > 30.0:
+                log(f"bench:   chain at {len(S) * every} sweeps: tau_int(S) {tau * every:.1f} +- {err * every:.1f}")
+                last = now
+        E1, S1 = lat.measure_series(per, every=every)
+        Es.append(E1)
+        Ss.append(S1)
+    E = np.concatenate(Es)
+    tauE, _ = tau_int(E)
+    tau_s = max(tau, tauE)  # samples; for the error of the means
+    n = len(S)
+    return {"every": every, "samples": n, "sweeps": n * every, "tau_int_S": tau * every, "tau_err": err * every,
+            "rel_err": err / tau, "resolved": bool(err <= target * tau), "window": M * every,
+            "E": float(E.mean()), "S": float(S.mean()),
+            "E_sem": float(E.std(ddof=1) * np.sqrt(2.0 * tau_s / n)),
+            "S_sem": float(S.std(ddof=1) * np.sqrt(2.0 * tau_s / n)), "seconds": time.perf_counter() - t0}
+
+
+def learned_leg(budget_s=30.0, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40, schedule=None,
+                target=0.2):
     """Config C5: 128^3 LL near the transition.  The network is trained on
     n_train neighbourhoods drawn from plain-sampler lattices at the three
     temperatures, then plain and learned chains start from the same
     equilibrated state; tau_int(S) by automatic windowing gives decorrelated
-    sweeps/s = sweeps/s / (2 tau_int)."""
+    sweeps/s = sweeps/s / (2 tau_int).  Each chain runs until tau_int(S) is
+    known to `target` relative error or `budget_s` seconds are spent; the gain carries a 95 %
+    interval from the two relative errors, and E and S of the two chains are
+    compared at 3 sigma (standard errors of the means from tau_int)."""
     import mclics
-    from mclics.decorrelation import tau_error, tau_int
     nt = L5**3
     out = {"workload": f"LL bulk {L5}^3 periodic, A=1, T in {list(temps)}", "n_train": n_train,
-           "measured_sweeps": n_meas, "per_T": []}
+           "budget_s_per_chain": budget_s, "tau_rel_err_target": target, "per_T": []}
     starts = {}
     with mclics.Network(seed=5) as net:
         per_T = n_train // len(temps)
@@ -104,6 +148,7 @@ def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
                 lat.sweeps(50, graph=True)
                 net.add_samples(lat, per_T // snaps + (n_train - per_T * len(temps) if ti == 0 and k == 0 else 0),
                                 seed=1000 * ti + k)
+            lat.sweeps(20000, graph=True)  # further equilibration of the chains' common start
             starts[T] = lat.directors()
             lat.close()
         # (epochs, learning rate) stages; Adam's moments and step carry across stages
@@ -126,17 +171,26 @@ def learned_leg(n_meas=100000, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
                         lat.sweeps(200, graph=True)
                         ms, _ = lat.time_sweeps(1000, graph=True)
                         sps = 1000 / (ms / 1e3)
-                    E, S = lat.measure_series(n_meas)
+                    st = chain_stats(lat, 1e6 / sps, budget_s, target)
                     acc, _ = lat.acceptance()
-                tau, M = tau_int(S)
-                row[method] = {"sweeps_per_s": sps, "tau_int_S": tau, "tau_err": tau_error(tau, M, len(S)),
-                               "window": M, "decorrelated_sweeps_per_s": sps / (2 * tau), "E": float(E.mean()),
-                               "S": float(S.mean()), "acceptance": acc / nt}
-                log(f"bench: learned leg T={T} {method}: {sps:.1f} sweeps/s tau={tau:.1f} acc={acc / nt:.3f}")
-            row["gain"] = row["learned"]["decorrelated_sweeps_per_s"] / row["plain"]["decorrelated_sweeps_per_s"]
-            flops = 11520.0 * nt * row["learned"]["sweeps_per_s"]
-            row["learned"]["mlp_tflops"] = flops / 1e12
-            row["learned"]["mfma_util_bf16_dense"] = flops / 2.5e15
+                st.update({"sweeps_per_s": sps, "decorrelated_sweeps_per_s": sps / (2 * st["tau_int_S"]),
+                           "acceptance": acc / nt})
+                row[method] = st
+                log(f"bench: learned leg T={T} {method}: {sps:.1f} sweeps/s tau={st['tau_int_S']:.1f} "
+                    f"+- {st['tau_err']:.1f} over {st['sweeps']} sweeps (every {st['every']}) acc={acc / nt:.3f}")
+            p, q = row["plain"], row["learned"]
+            row["gain"] = q["decorrelated_sweeps_per_s"] / p["decorrelated_sweeps_per_s"]
+            rel = float(np.hypot(p["rel_err"], q["rel_err"]))
+            row["gain_95"] = [row["gain"] * float(np.exp(-1.96 * rel)), row["gain"] * float(np.exp(1.96 * rel))]
+            row["resolved"] = p["resolved"] and q["resolved"]
+            for what in ("E", "S"):
+                d = abs(p[what] - q[what])
+                sig = float(np.hypot(p[what + "_sem"], q[what + "_sem"]))
+                row[what + "_agree_3sigma"] = bool(d <= 3 * sig)
+                row[what + "_diff_over_sigma"] = d / sig if sig > 0 else float("inf")
+            flops = 11520.0 * nt * q["sweeps_per_s"]
+            q["mlp_tflops"] = flops / 1e12
+            q["mfma_util_bf16_dense"] = flops / 2.5e15
             gains.append(row["gain"])
             out["per_T"].append(row)
         out["gain_geomean"] = float(np.exp(np.mean(np.log(gains))))
@@ -326,8 +380,11 @@ def main():
     ap.add_argument("--no-slab-geometry", action="store_true", help="skip the GHRL slab-geometry leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the 30^3 thermal run")
     ap.add_argument("--learned-epochs", type=int, default=40, help="training epochs of the learned-proposal leg")
-    ap.add_argument("--learned-sweeps", type=int, default=100000,
-                    help="measured sweeps per chain in the learned-proposal leg")
+    ap.add_argument("--learned-budget", type=float, default=30.0,
+                    help="seconds per chain of the learned-proposal leg (each chain stops earlier once "
+                         "tau_int(S) is known to --learned-target)")
+    ap.add_argument("--learned-target", type=float, default=0.2,
+                    help="relative error of tau_int(S) at which a chain of the learned-proposal leg stops")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
                     help="sweep kernel of the headline leg (auto: slab where it fits)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -444,7 +501,8 @@ def main():
         result["slab_geometry_cholesteric"] = slab_geometry_leg(p0=16.0)
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
-        result["learned_proposal"] = learned_leg(args.learned_sweeps, epochs=args.learned_epochs)
+        result["learned_proposal"] = learned_leg(args.learned_budget, epochs=args.learned_epochs,
+                                                 target=args.learned_target)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         result["cpu_baseline"] = cpu_baseline(n0)

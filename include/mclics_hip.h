@@ -163,6 +163,10 @@ int mc_measure(mc_handle *h, double *E, double *S, double *q6);
  * the per-sweep values.  This is the MCS loop of the mode drivers
  * (src/evolve_thermal.cu:85-95) with the lattice never leaving the device. */
 int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series);
+/* The same with `every` sweeps before each measurement (every = 1 is
+ * mc_measure_series): thinned series for long autocorrelation studies.  No
+ * reference counterpart: the reference measures after every MCS sweep. */
+int mc_measure_series_every(mc_handle *h, int n, int every, double *E_series, double *S_series);
 /* Allocation-free measurement.  No counterpart in the reference: an entry
  * point for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It

@@ -1004,9 +1004,13 @@ int mc_measure(mc_handle *h, double *E, double *S, double *q6) {
 }
 
 int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series) {
+  return mc_measure_series_every(h, n, 1, E_series, S_series);
+}
+
+int mc_measure_series_every(mc_handle *h, int n, int every, double *E_series, double *S_series) {
   int rc = check_handle(h);
   if (rc) return rc;
-  if (n < 1) return fail(MC_EINVAL, "mc_measure_series: need n >= 1");
+  if (n < 1 || every < 1) return fail(MC_EINVAL, "mc_measure_series: need n >= 1 and every >= 1");
   if (n > h->series_cap) {
     if (h->series) MC_HIP(hipFree(h->series));
     h->series = nullptr;
@@ -1015,8 +1019,10 @@ int mc_measure_series(mc_handle *h, int n, double *E_series, double *S_series) {
     h->series_cap = n;
   }
   for (int w = 0; w < n; ++w) {
-    rc = enqueue_sweep(h, nullptr);
-    if (rc) return rc;
+    for (int e = 0; e < every; ++e) {
+      rc = enqueue_sweep(h, nullptr);
+      if (rc) return rc;
+    }
     // (re-read after the sweep: the slab kernel moves the lattice to the other plane set)
     launch_measure(h, h->series + 2 * (size_t)w);
   }

@@ -51,6 +51,7 @@ EXPORTS = (
     "mc_time_passes",
     "mc_measure",
     "mc_measure_series",
+    "mc_measure_series_every",
     "mc_measure_noalloc",
     "mc_site_energies",
     "mc_local_order",
@@ -167,6 +168,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                    C.POINTER(C.c_int)]
     lib.mc_measure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_measure_series.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.mc_measure_series_every.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.mc_measure_noalloc.argtypes = [C.POINTER(McParams), vp, vp, vp, vp, vp]
     lib.mc_site_energies.argtypes = [vp, f32p]
     lib.mc_local_order.argtypes = [vp, f32p]
@@ -419,11 +421,12 @@ class Lattice:
         _check(self._lib.mc_measure(self._h, C.byref(E), C.byref(S), q))
         return E.value, S.value, np.array(q[:], np.float64)
 
-    def measure_series(self, n: int):
+    def measure_series(self, n: int, every: int = 1):
+        """n measurements (E, S), each after `every` sweeps."""
         E = np.empty(n, np.float64)
         S = np.empty(n, np.float64)
-        _check(self._lib.mc_measure_series(self._h, n, E.ctypes.data_as(C.POINTER(C.c_double)),
-                                           S.ctypes.data_as(C.POINTER(C.c_double))))
+        _check(self._lib.mc_measure_series_every(self._h, n, every, E.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 S.ctypes.data_as(C.POINTER(C.c_double))))
         return E, S
 
     def site_energies(self) -> np.ndarray:

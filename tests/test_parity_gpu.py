@@ -684,6 +684,26 @@ def test_slab_measure_series_reads_the_current_planes():
             assert E[w] == e and S[w] == s_, w
 
 
+@pytest.mark.parametrize("kernel", ["slab", "x4", "generic"])
+def test_measure_series_every_equals_sweeps_then_measure(kernel):
+    # mc_measure_series_every(n, k): k sweeps before each of n measurements,
+    # odd k so the slab kernel's plane set alternates between samples.
+    nx, ny, nz = 16, 8, 8
+    n = unit(np.random.default_rng(23), nx * ny * nz)
+    cfg = mclics.Config(nx, ny, nz, T=1.05, seed=13)
+    kind = {"slab": mclics.KERNEL_SLAB, "x4": mclics.KERNEL_X4, "generic": mclics.KERNEL_GENERIC}[kernel]
+    with mclics.Lattice(cfg, n) as a, mclics.Lattice(cfg, n) as b:
+        a.set_kernel(kind)
+        b.set_kernel(kind)
+        E, S = a.measure_series(6, every=3)
+        for w in range(6):
+            b.sweeps(3)
+            e, s_, _ = b.measure()
+            assert E[w] == e and S[w] == s_, (kernel, w)
+        assert a.acceptance() == b.acceptance() and a.ang_var() == b.ang_var()
+        assert np.array_equal(a.directors(), b.directors())
+
+
 @pytest.mark.parametrize("kernel", ["slab", "generic", "ref8"])
 def test_checkpoint_resumes_the_chain_exactly(kernel):
     # 12 sweeps in one go equal 5 sweeps, a checkpoint (directors, ang_var,
