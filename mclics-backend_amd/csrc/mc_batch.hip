@@ -188,19 +188,6 @@ bool slab_plan(const mc_batch *b, int budget, int cz_target, bool aligned, std::
       for (int y0 = 0; y0 < P.ny; y0 += ty) wg.push_back(int4{r, y0, z0, 0});
   }
   *threads = (*threads + 63) / 64 * 64;
-  // Longest tiles first.  The hardware hands out workgroups in index order
-  // as CUs free up, so this is the longest-processing-time-first schedule:
-  // the short tiles of the small replicas fill in at the end instead of the
-  // large ones starting last.  Cost: the red updates of the TY+2 rows of
-  // CZ+2 planes and the black ones of TY rows of CZ planes, per segment of a
-  // row.  (Nothing depends on the order: each workgroup reads its own entry,
-  // and a replica's last workgroup is found by counting.)
-  std::vector<long long> cost(b->nrep);
-  for (int r = 0; r < b->nrep; ++r) {
-    const long long S = b->hp[r].nx / 4, ty = geo[r].x, cz = geo[r].y;
-    cost[r] = S * ((ty + 2) * (cz + 2) + ty * cz);
-  }
-  std::stable_sort(wg.begin(), wg.end(), [&](const int4 &a, const int4 &c) { return cost[a.x] > cost[c.x]; });
   return true;
 }
 
