@@ -56,7 +56,13 @@ enum mc_order { MC_ORDER_CHECKER = 0, MC_ORDER_REF8 = 1 };
  * evolve the lattice identically; the generic kernel uses another counter
  * layout of the generator, so it is a different (equally valid) chain from
  * the same seed. */
-enum mc_kernel { MC_KERNEL_AUTO = 0, MC_KERNEL_GENERIC = 1, MC_KERNEL_X4 = 2, MC_KERNEL_SLAB = 3 };
+enum mc_kernel {
+  MC_KERNEL_AUTO = 0,
+  MC_KERNEL_GENERIC = 1,
+  MC_KERNEL_X4 = 2,
+  MC_KERNEL_SLAB = 3,
+  MC_KERNEL_SLAB_PERSISTENT = 4 /* batches only: up to 64 sweeps per launch */
+};
 
 /* POD flattening of the reference's Parameters (include/parameters.h:11-83)
  * restricted to what the sweep reads.  The std::map surface tables become
@@ -253,10 +259,19 @@ int mc_batch_get_state(mc_batch *b, uint32_t *accepted, uint32_t *sweeps, float 
 /* Segment-kernel launch shape (grid 0 = default); selects the segment kernel. */
 int mc_batch_set_tuning(mc_batch *b, int grid, int xcd);
 /* Sweep kernel of the batch: MC_KERNEL_AUTO (the slab kernel where every
- * replica has a tile shape, else the segment kernel), MC_KERNEL_SLAB or
- * MC_KERNEL_X4.  Both give every replica the same chain. */
+ * replica has a tile shape, else the segment kernel), MC_KERNEL_SLAB,
+ * MC_KERNEL_SLAB_PERSISTENT or MC_KERNEL_X4.  All give every replica the
+ * same chain.  MC_KERNEL_SLAB_PERSISTENT runs the slab tiles of up to 64
+ * sweeps in one launch: as many workgroups as the device holds at once take
+ * tiles from a per-sweep counter, with a grid-wide barrier between sweeps.
+ * Its waits are bounded (mc_batch_set_barrier_limit): a barrier that does
+ * not complete sets a fault that the next synchronising call (mc_batch_sync,
+ * measurements, timing, state) reports as MC_EHIP. */
 int mc_batch_set_kernel(mc_batch *b, int kind);
 int mc_batch_get_kernel(mc_batch *b, int *kind);
+/* Polls of a persistent launch's grid barrier before it gives up (default
+ * 2^21, seconds of waiting; tests lower it to force the fault path). */
+int mc_batch_set_barrier_limit(mc_batch *b, unsigned int polls);
 /* Times segment grids and slab shapes on scratch copies and keeps the
  * fastest; grid = its workgroup count, xcd its tile schedule. */
 int mc_batch_autotune(mc_batch *b, int *grid, int *xcd, float *best_us);

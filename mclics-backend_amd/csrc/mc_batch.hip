@@ -76,6 +76,13 @@ struct mc_batch {
   // shape, and the shape's parameters (thread budget, planes per tile).
   int kind = MC_KERNEL_AUTO;  // requested sweep kernel
   bool slab = false;          // active: slab (true) or segment (false)
+  // Persistent slab sweeps (MC_KERNEL_SLAB_PERSISTENT): up to kPersistChunk
+  // sweeps per launch of persist_grid workgroups, with the launch's barrier
+  // and tile counters (and the fault flag, word 1) in d_sync.
+  bool persist = false;
+  int persist_grid = 0;
+  unsigned int *d_sync = nullptr;
+  unsigned int spin_limit = 1u << 21;
   float *bx = nullptr, *by = nullptr, *bz = nullptr;
   int flip = 0;
   int4 *d_wg = nullptr;
@@ -124,7 +131,7 @@ void free_batch(mc_batch *b) {
   for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   void *ptrs[] = {b->d_params, b->d_states, b->d_tile_rep, b->d_rep_tile0, b->d_rep_ntiles, b->d_rep_off, b->nx,
                   b->ny, b->nz, b->d_done, b->d_partials, b->d_out, b->d_series, b->staging, b->bx, b->by, b->bz,
-                  b->d_wg, b->d_geo, b->d_mwg, b->d_mgeo, b->d_mwg0, b->d_mnwg};
+                  b->d_wg, b->d_geo, b->d_mwg, b->d_mgeo, b->d_mwg0, b->d_mnwg, b->d_sync};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -238,6 +245,8 @@ void launch_batch_measure(mc_batch *b, double *series) {
     mc_launch_batch_measure(b->launch(), b->cu_count * 8, b->d_partials, b->d_out, series);
 }
 
+int persist_prepare(mc_batch *b);
+
 // Makes (budget, cz_target, aligned) the slab shape: uploads the tables.
 int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
   std::vector<int4> wg;
@@ -257,6 +266,22 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
   b->slab_cz = cz_target;
   b->slab_aligned = aligned;
   (void)mc_slab_prepare(lds);
+  return b->persist ? persist_prepare(b) : MC_OK;
+}
+
+// Sweeps per persistent launch (the graphs hold at most this many sweeps).
+constexpr int kPersistChunk = 64;
+
+// The persistent kernel's grid (what the device holds at once at the current
+// slab shape, at most one workgroup per tile) and its sync words.
+int persist_prepare(mc_batch *b) {
+  const int resident = mc_batch_persist_grid(b->potential, b->S.threads, b->S.lds, b->cu_count);
+  if (resident < 1) return bfail(MC_EINVAL, "persistent batch sweep: the slab tile does not fit a CU");
+  b->persist_grid = std::min(resident, b->S.nwg);
+  if (!b->d_sync) {
+    MB_HIP(hipMalloc(&b->d_sync, (kPersistChunk + 2) * sizeof(unsigned int)));
+    MB_HIP(hipMemset(b->d_sync, 0, (kPersistChunk + 2) * sizeof(unsigned int)));
+  }
   return MC_OK;
 }
 
@@ -265,6 +290,8 @@ int set_slab_shape(mc_batch *b, int budget, int cz_target, bool aligned) {
 // AUTO takes the slab for Lebwohl-Lasher only (GHRL and pear evaluate every
 // move exactly, which the segment kernel's occupancy serves better).
 int select_kernel(mc_batch *b, int kind) {
+  const bool persist = kind == MC_KERNEL_SLAB_PERSISTENT;
+  if (persist) kind = MC_KERNEL_SLAB;
   bool want = kind == MC_KERNEL_SLAB || (kind == MC_KERNEL_AUTO && b->potential == MCD_LL);
   if (want && b->slab_budget == 0) {
     std::vector<int4> wg;
@@ -298,21 +325,50 @@ int select_kernel(mc_batch *b, int kind) {
   if (kind == MC_KERNEL_SLAB && !want)
     return bfail(MC_EINVAL, "mc_batch_set_kernel: no slab tile shape fits every replica (or no memory for the "
                             "second plane set)");
-  b->kind = kind;
+  if (persist) {
+    const int rc = persist_prepare(b);
+    if (rc) return rc;
+  }
+  b->kind = persist ? MC_KERNEL_SLAB_PERSISTENT : kind;
   b->slab = want;
+  b->persist = persist;
   drop_graphs(b);
   return MC_OK;
 }
 
-void enqueue_batch_sweep(mc_batch *b) {
-  const MCBatchLaunch B = b->launch();
-  if (b->slab) {
-    mc_launch_batch_slab(B, b->S, b->bx, b->by, b->bz);
-    swap_planes(b);
+// n sweeps (n <= kPersistChunk for the persistent kernel).
+void enqueue_batch_sweeps(mc_batch *b, int n) {
+  if (b->slab && b->persist) {
+    mc_launch_batch_slab_persist(b->launch(), b->S, b->bx, b->by, b->bz, b->persist_grid, n, b->d_sync,
+                                 b->spin_limit);
+    if (n & 1) swap_planes(b);
     return;
   }
-  mc_launch_batch_x4(B, 0, b->sweep_grid(), 0);
-  mc_launch_batch_x4(B, 1, b->sweep_grid(), 1);
+  for (int w = 0; w < n; ++w) {
+    const MCBatchLaunch B = b->launch();
+    if (b->slab) {
+      mc_launch_batch_slab(B, b->S, b->bx, b->by, b->bz);
+      swap_planes(b);
+      continue;
+    }
+    mc_launch_batch_x4(B, 0, b->sweep_grid(), 0);
+    mc_launch_batch_x4(B, 1, b->sweep_grid(), 1);
+  }
+}
+
+void enqueue_batch_sweep(mc_batch *b) { enqueue_batch_sweeps(b, 1); }
+
+// Reports (and clears) a timed-out grid barrier of a persistent launch; the
+// stream must have drained.
+int check_fault(mc_batch *b) {
+  if (!b->d_sync) return MC_OK;
+  unsigned int f = 0;
+  MB_HIP(hipMemcpy(&f, b->d_sync + 1, sizeof(f), hipMemcpyDeviceToHost));
+  if (!f) return MC_OK;
+  const unsigned int zero = 0;
+  MB_HIP(hipMemcpy(b->d_sync + 1, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  return bfail(MC_EHIP, "persistent batch sweep: a grid barrier did not complete (workgroups not resident at once, "
+                        "or the wait limit too small); the lattices are not valid");
 }
 
 // Graph of n sweeps from the current plane set.  Capture records launches
@@ -325,7 +381,7 @@ int get_graph(mc_batch *b, int n, hipGraphExec_t *out) {
     const int flip0 = b->flip;
     hipGraph_t g = nullptr;
     MB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int w = 0; w < n; ++w) enqueue_batch_sweep(b);
+    enqueue_batch_sweeps(b, n);
     if (b->flip != flip0) swap_planes(b);
     MB_HIP(hipStreamEndCapture(b->stream, &g));
     hipGraphExec_t ge = nullptr;
@@ -350,6 +406,7 @@ int launch_graph(mc_batch *b, int n, hipGraphExec_t ge) {
 // remainder graph starts from the plane set it was captured on.
 constexpr int kGraphChunk = 64;
 static_assert(kGraphChunk % 2 == 0, "the chunk must keep the slab kernel's plane set");
+static_assert(kGraphChunk <= kPersistChunk, "a graph's sweeps must fit one persistent launch");
 
 int run_graphs(mc_batch *b, int n, bool prepare_only) {
   const int reps = n / kGraphChunk, rem = n % kGraphChunk;
@@ -538,7 +595,7 @@ int mc_batch_sync(mc_batch *b) {
   int rc = check_batch(b);
   if (rc) return rc;
   MB_HIP(hipStreamSynchronize(b->stream));
-  return MC_OK;
+  return check_fault(b);
 }
 
 int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
@@ -553,7 +610,7 @@ int mc_batch_time_sweeps(mc_batch *b, int n, float *ms) {
   MB_HIP(hipEventRecord(b->ev1, b->stream));
   MB_HIP(hipEventSynchronize(b->ev1));
   MB_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
-  return MC_OK;
+  return check_fault(b);
 }
 
 int mc_batch_measure(mc_batch *b, double *E, double *S) {
@@ -564,6 +621,8 @@ int mc_batch_measure(mc_batch *b, double *E, double *S) {
   std::vector<double> o((size_t)b->nrep * 8);
   MB_HIP(hipMemcpyAsync(o.data(), b->d_out, o.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   MB_HIP(hipStreamSynchronize(b->stream));
+  rc = check_fault(b);
+  if (rc) return rc;
   for (int r = 0; r < b->nrep; ++r) {
     if (E) E[r] = o[(size_t)r * 8];
     if (S) S[r] = o[(size_t)r * 8 + 1];
@@ -592,6 +651,8 @@ int mc_batch_measure_series(mc_batch *b, int n, double *E, double *S) {
   std::vector<double> o(need);
   MB_HIP(hipMemcpyAsync(o.data(), b->d_series, need * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   MB_HIP(hipStreamSynchronize(b->stream));
+  rc = check_fault(b);
+  if (rc) return rc;
   for (int w = 0; w < n; ++w)
     for (int r = 0; r < b->nrep; ++r) {
       if (E) E[(size_t)w * b->nrep + r] = o[2 * ((size_t)w * b->nrep + r)];
@@ -619,6 +680,8 @@ int mc_batch_get_state(mc_batch *b, uint32_t *accepted, uint32_t *sweeps, float 
   std::vector<MCDevState> st(b->nrep);
   MB_HIP(hipMemcpyAsync(st.data(), b->d_states, b->nrep * sizeof(MCDevState), hipMemcpyDeviceToHost, b->stream));
   MB_HIP(hipStreamSynchronize(b->stream));
+  rc = check_fault(b);
+  if (rc) return rc;
   for (int r = 0; r < b->nrep; ++r) {
     if (accepted) accepted[r] = st[r].last_acc;
     if (sweeps) sweeps[r] = st[r].sweep;
@@ -638,14 +701,16 @@ int mc_batch_set_tuning(mc_batch *b, int grid, int xcd) {
   b->xcd = xcd;
   b->kind = MC_KERNEL_X4;
   b->slab = false;
+  b->persist = false;
   return MC_OK;
 }
 
 int mc_batch_set_kernel(mc_batch *b, int kind) {
   int rc = check_batch(b);
   if (rc) return rc;
-  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_SLAB && kind != MC_KERNEL_X4)
-    return bfail(MC_EINVAL, "mc_batch_set_kernel: kind must be MC_KERNEL_AUTO, MC_KERNEL_SLAB or MC_KERNEL_X4");
+  if (kind != MC_KERNEL_AUTO && kind != MC_KERNEL_SLAB && kind != MC_KERNEL_X4 && kind != MC_KERNEL_SLAB_PERSISTENT)
+    return bfail(MC_EINVAL, "mc_batch_set_kernel: kind must be MC_KERNEL_AUTO, MC_KERNEL_SLAB, "
+                            "MC_KERNEL_SLAB_PERSISTENT or MC_KERNEL_X4");
   MB_HIP(hipStreamSynchronize(b->stream));
   return select_kernel(b, kind);
 }
@@ -654,7 +719,17 @@ int mc_batch_get_kernel(mc_batch *b, int *kind) {
   int rc = check_batch(b);
   if (rc) return rc;
   if (!kind) return bfail(MC_EINVAL, "mc_batch_get_kernel: null output");
-  *kind = b->slab ? MC_KERNEL_SLAB : MC_KERNEL_X4;
+  *kind = b->slab ? (b->persist ? MC_KERNEL_SLAB_PERSISTENT : MC_KERNEL_SLAB) : MC_KERNEL_X4;
+  return MC_OK;
+}
+
+int mc_batch_set_barrier_limit(mc_batch *b, unsigned int polls) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (polls < 1) return bfail(MC_EINVAL, "mc_batch_set_barrier_limit: need at least one poll");
+  MB_HIP(hipStreamSynchronize(b->stream));
+  drop_graphs(b);
+  b->spin_limit = polls;
   return MC_OK;
 }
 

@@ -89,6 +89,14 @@ struct MCBatchSlab {
   size_t lds;
 };
 void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *ox, float *oy, float *oz);
+// Persistent form: nsweeps sweeps in one launch of `grid` workgroups (at most
+// what the device holds at once: mc_batch_persist_grid), reading B.nx.. in
+// even sweeps and o* in odd ones; sync is a device array of nsweeps + 2 words
+// (word 1: fault flag, set when a grid barrier timed out after spin_limit
+// polls).  The reset of the other words is queued before the launch.
+int mc_batch_persist_grid(int potential, int threads, size_t lds, int cu_count);
+void mc_launch_batch_slab_persist(const MCBatchLaunch &B, const MCBatchSlab &S, float *ox, float *oy, float *oz,
+                                  int grid, int nsweeps, unsigned int *sync, unsigned int spin_limit);
 // Per-replica FP64 {E, Qxx..Qzz, n} partial sums per tile, then per-replica
 // reduction in tile order: out[r*8..] = {E_mean, S, q6}.
 void mc_launch_batch_measure(const MCBatchLaunch &B, int grid, double *tile_partials, double *out, double *series);

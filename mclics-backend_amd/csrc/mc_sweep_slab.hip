@@ -346,6 +346,99 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch_slab(const MCDevParams *_
   if (threadIdx.x == 0) mcd_finish_replica(P, st, bsum, (uint32_t)((P.ny / sh.x) * (P.nz / sh.y)));
 }
 
+// Persistent batched sweeps: one launch runs nsweeps sweeps of every replica.
+// Workgroups take tiles from one counter per sweep; a grid-wide barrier
+// separates the sweeps, so a sweep reads the plane set the previous one
+// wrote (sweep w reads a when w is even, b when odd, and writes the other).
+// sync[0] counts barrier arrivals over the launch, sync[1] is the fault flag
+// (left set for the host), sync[2 + w] is the tile counter of sweep w; the
+// host zeroes sync[0] and the counters before the launch.  The grid must fit
+// the device at once (the host sizes it from the occupancy); even so every
+// wait is bounded: a barrier that has not completed after spin_limit polls
+// sets the fault flag and its workgroup returns, so the launch always ends.
+//
+// Tile fetch: thread 0 takes a ticket into one of two LDS slots, a barrier
+// publishes it, and every thread reads the same value, so the whole
+// workgroup leaves the loop together.  The slot a thread reads is written
+// again only two fetches later, with the tile's own barriers in between.
+//
+// Memory order across workgroups (and across XCDs, whose L2 caches are not
+// coherent with each other): the workgroup barrier orders every wave's
+// stores before thread 0, whose device-scope release fence then writes them
+// back before it arrives; after the barrier has completed, thread 0's
+// device-scope acquire fence invalidates the caches the workgroup reads
+// through, before any wave reads the planes, the replicas' states (ang_var,
+// sweep index) or the tile counters again.  (One release and one acquire per
+// workgroup and sweep: a device-scope fence costs an L2 write-back.)  A
+// workgroup also leaves the wait when another has already flagged a fault.
+__device__ __forceinline__ bool grid_barrier(unsigned int *sync, unsigned int target, unsigned int spin_limit,
+                                             int *ok_slot) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned int n = 0;
+    int ok = 1;
+    while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++n >= spin_limit || __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        __hip_atomic_fetch_or(sync + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(32);  // about 2,000 cycles: pollers of one word serialise at its channel
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *ok_slot = ok;
+  }
+  __syncthreads();
+  const bool ok = *ok_slot != 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return ok;
+}
+
+template <int POT>
+__global__ __launch_bounds__(kMaxThreads) void k_batch_slab_persist(
+    const MCDevParams *__restrict__ params, MCDevState *states, const int4 *__restrict__ wg,
+    const int2 *__restrict__ geo, const long long *__restrict__ rep_off, int ntiles, float *ax, float *ay, float *az,
+    float *bx, float *by, float *bz, int nsweeps, unsigned int *sync, unsigned int spin_limit) {
+  __shared__ int slot[2];
+  __shared__ int ok_slot;
+  int k = 0;
+  for (int w = 0; w < nsweeps; ++w) {
+    const bool odd = (w & 1) != 0;
+    float *const ix = odd ? bx : ax, *const iy = odd ? by : ay, *const iz = odd ? bz : az;
+    float *const ox = odd ? ax : bx, *const oy = odd ? ay : by, *const oz = odd ? az : bz;
+    for (;;) {
+      if (threadIdx.x == 0)
+        slot[k & 1] = (int)__hip_atomic_fetch_add(sync + 2 + w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      const int t = slot[k & 1];
+      ++k;
+      if (t >= ntiles) break;
+      const int4 d = wg[t];
+      const int r = d.x;
+      const MCDevParams P = params[r];
+      MCDevState *st = states + r;
+      const int2 sh = geo[r];
+      const SlabGeom g = slab_geom(P, sh.x, sh.y);
+      const long long off = rep_off[r];
+      const float *const in[3] = {ix + off, iy + off, iz + off};
+      float *const out[3] = {ox + off, oy + off, oz + off};
+      const mcsu::PassConst kc = mcsu::pass_const(P, st);
+      const uint32_t acc = slab_tile<POT, false, false>(P, kc, g, in, out, d.y, d.z, MCRecord{});
+      const uint32_t bsum = slab_block_sum(acc);
+      if (threadIdx.x == 0) mcd_finish_replica(P, st, bsum, (uint32_t)((P.ny / sh.x) * (P.nz / sh.y)));
+    }
+    if (w + 1 < nsweeps && !grid_barrier(sync, (unsigned int)(w + 1) * gridDim.x, spin_limit, &ok_slot)) return;
+  }
+}
+
+__global__ void k_persist_reset(unsigned int *sync, int nsweeps) {
+  for (int i = threadIdx.x; i < nsweeps + 2; i += blockDim.x)
+    if (i != 1) sync[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Measurement on the slab layout: mean site energy (each site's energy with
 // the reference arithmetic, src/geometry.cpp:19-72: FP32 bonds summed in
@@ -704,6 +797,9 @@ int mc_slab_prepare(size_t lds_bytes) {
       reinterpret_cast<const void *>(&k_batch_slab<MCD_LL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_GHRL>),
       reinterpret_cast<const void *>(&k_batch_slab<MCD_PEAR>),
+      reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_LL>),
+      reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_GHRL>),
+      reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_PEAR>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false, 0, 0>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, false, false, kFixedRowSegs, kFixedTileRows>),
       reinterpret_cast<const void *>(&k_sweep_slab<MCD_LL, true, false, 0, 0>),
@@ -778,4 +874,37 @@ void mc_launch_batch_slab(const MCBatchLaunch &B, const MCBatchSlab &S, float *o
       break;
   }
 #undef MC_BSLAB
+}
+
+int mc_batch_persist_grid(int potential, int threads, size_t lds, int cu_count) {
+  const void *fn = potential == MCD_GHRL ? reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_GHRL>)
+                   : potential == MCD_PEAR ? reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_PEAR>)
+                                           : reinterpret_cast<const void *>(&k_batch_slab_persist<MCD_LL>);
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return per * cu_count;
+}
+
+void mc_launch_batch_slab_persist(const MCBatchLaunch &B, const MCBatchSlab &S, float *ox, float *oy, float *oz,
+                                  int grid, int nsweeps, unsigned int *sync, unsigned int spin_limit) {
+  k_persist_reset<<<1, 64, 0, B.stream>>>(sync, nsweeps);
+#define MC_PSLAB(POT_)                                                                                         \
+  k_batch_slab_persist<POT_><<<grid, S.threads, S.lds, B.stream>>>(B.params, B.states, S.wg, S.geo, B.rep_off, \
+                                                                   S.nwg, B.nx, B.ny, B.nz, ox, oy, oz, nsweeps, \
+                                                                   sync, spin_limit)
+  switch (B.potential) {
+    case MCD_GHRL:
+      MC_PSLAB(MCD_GHRL);
+      break;
+    case MCD_PEAR:
+      MC_PSLAB(MCD_PEAR);
+      break;
+    default:
+      MC_PSLAB(MCD_LL);
+      break;
+  }
+#undef MC_PSLAB
 }

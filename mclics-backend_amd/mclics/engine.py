@@ -20,7 +20,7 @@ MC_MAX_SURF = 8
 FREE, PERIODIC = 0, 1
 LL, GHRL, PEAR = 0, 1, 2
 ORDER_CHECKER, ORDER_REF8 = 0, 1
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_X4, KERNEL_SLAB = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_X4, KERNEL_SLAB, KERNEL_SLAB_PERSISTENT = 0, 1, 2, 3, 4
 ANCHORING = {
     "homeotropic": 0,
     "rp": 1,
@@ -73,6 +73,7 @@ EXPORTS = (
     "mc_batch_time_sweeps",
     "mc_batch_measure",
     "mc_batch_measure_series",
+    "mc_batch_set_barrier_limit",
     "mc_batch_get_directors_aos",
     "mc_batch_get_state",
     "mc_batch_set_tuning",
@@ -193,6 +194,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_batch_time_sweeps.argtypes = [vp, C.c_int, f32p]
     lib.mc_batch_measure.argtypes = [vp, dp, dp]
     lib.mc_batch_measure_series.argtypes = [vp, C.c_int, dp, dp]
+    lib.mc_batch_set_barrier_limit.argtypes = [vp, C.c_uint]
     lib.mc_batch_get_directors_aos.argtypes = [vp, C.c_int, f32p]
     lib.mc_batch_get_state.argtypes = [vp, u32p, u32p, f32p]
     lib.mc_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
@@ -542,6 +544,9 @@ class Batch:
 
     def set_tuning(self, grid: int, xcd: int):
         _check(self._lib.mc_batch_set_tuning(self._h, grid, xcd))
+
+    def set_barrier_limit(self, polls: int):
+        _check(self._lib.mc_batch_set_barrier_limit(self._h, polls))
 
     def set_kernel(self, kind: int) -> None:
         _check(self._lib.mc_batch_set_kernel(self._h, kind))

@@ -161,3 +161,53 @@ def test_batch_rejects_unsupported_replicas():
         mclics.Batch([mclics.Config(6, 4, 4)], [mclics.random_ic(6, 4, 4, seed=1)])  # Nx % 4
     with pytest.raises(mclics.EngineError):
         mclics.Batch([mclics.Config(8, 4, 4, bound=(1, 1, 0))], [mclics.random_ic(8, 4, 4, seed=1)])
+
+
+def test_batch_persistent_launch_keeps_the_chain():
+    """MC_KERNEL_SLAB_PERSISTENT (up to 64 sweeps per launch, tiles taken
+    from a per-sweep counter, a grid barrier between sweeps) gives every
+    replica the chain of the one-launch-per-sweep slab kernel: runs of odd
+    and even length, one across the 64-sweep chunk, and measured sweeps."""
+    sizes = [(32, 32, 32), (8, 8, 8), (12, 6, 10), (16, 16, 16), (20, 8, 8), (24, 12, 16)]
+    rng = np.random.default_rng(29)
+    cfgs, ics = [], []
+    for r, (x, y, z) in enumerate(sizes):
+        T = float(np.float32(rng.uniform(0.9, 1.3)))
+        cfgs.append(mclics.Config(x, y, z, T=T, seed=600 + r, stream=r))
+        ics.append(mclics.random_ic(x, y, z, seed=600 + r))
+    with mclics.Batch(cfgs, ics) as b1, mclics.Batch(cfgs, ics) as b2:
+        b1.set_kernel(mclics.KERNEL_SLAB)
+        b2.set_kernel(mclics.KERNEL_SLAB_PERSISTENT)
+        assert b2.kernel() == mclics.KERNEL_SLAB_PERSISTENT
+        for n in (3, 4, 1, 67):
+            print(f"persistent batch: {n} sweeps", flush=True)
+            b1.sweeps(n)
+            b2.sweeps(n)
+        E1, S1 = b1.measure_series(5)
+        E2, S2 = b2.measure_series(5)
+        assert np.array_equal(E1, E2) and np.array_equal(S1, S2)
+        for r in range(len(sizes)):
+            assert np.array_equal(b1.directors(r), b2.directors(r)), r
+        a1, s1, v1 = b1.state()
+        a2, s2, v2 = b2.state()
+        assert np.array_equal(a1, a2) and np.array_equal(s1, s2) and np.array_equal(v1, v2)
+        assert np.all(s2 == 80)
+        ms = b2.time_sweeps(20)
+        assert ms > 0
+
+
+def test_batch_persistent_barrier_timeout_is_reported():
+    """A grid barrier that gives up (forced from the host with a one-poll
+    limit) ends the launch and is reported as an error by the next
+    synchronising call; the kernel's waits are bounded, so nothing hangs."""
+    sizes = [(16, 16, 16), (32, 32, 32)]
+    cfgs = [mclics.Config(*d, T=1.1, seed=700 + r, stream=r) for r, d in enumerate(sizes)]
+    ics = [mclics.random_ic(*d, seed=700 + r) for r, d in enumerate(sizes)]
+    with mclics.Batch(cfgs, ics) as b:
+        b.set_kernel(mclics.KERNEL_SLAB_PERSISTENT)
+        b.set_barrier_limit(1)
+        b.sweeps(8)
+        with pytest.raises(mclics.EngineError) as ei:
+            b.sync()
+        assert ei.value.code == -2 and "grid barrier" in str(ei.value)
+        b.sync()  # the fault is reported once
