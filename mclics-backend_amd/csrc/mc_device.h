@@ -326,6 +326,18 @@ __device__ __forceinline__ float mcd_bond(const MCDevParams &P, double cA, int b
 // relative error of exp2(x*log2e) under 2^-12, and glibc's expf is within
 // one ulp of exp); the rare near-ties, u == 0 and infinities are settled
 // with mcx_expf, which returns glibc's expf bit for bit (mc_expf.h).
+// mcx_expf's table in device global memory (one copy per translation unit).
+static __device__ const uint64_t mcd_exp_table[32] = MCX_EXPF_TABLE;
+
+// tab: the 32 entries of mcx_expf's table, wherever the caller keeps them
+// (a kernel that must not wait on vector memory in this rare path copies
+// them into LDS).
+__device__ __forceinline__ bool mcd_accept_with(float E_new, float E_old, float T, float u, const uint64_t *tab) {
+  float x = -(E_new - E_old) / T;
+  float pf = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  if (u != 0.0f && !(fabsf(u - pf) <= pf * 2.44140625e-4f)) return u < pf;
+  return u < mcx_expf_with(x, tab);
+}
 __device__ __forceinline__ bool mcd_accept(float E_new, float E_old, float T, float u) {
   float x = -(E_new - E_old) / T;
   float pf = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);

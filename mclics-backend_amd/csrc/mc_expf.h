@@ -39,16 +39,25 @@ MCX_HD inline double mcx_f64_from(uint64_t u) {
   return d;
 }
 
-MCX_HD inline float mcx_expf(float x) {
-  constexpr uint64_t kTab[32] = {
-      0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
-      0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
-      0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
-      0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
-      0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
-      0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
-      0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
-      0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+#define MCX_EXPF_TABLE {\
+      0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,\
+      0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,\
+      0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,\
+      0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,\
+      0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,\
+      0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,\
+      0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,\
+      0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull}
+
+// Entry i of the table (for copying it into fast memory).
+MCX_HD inline uint64_t mcx_expf_table_entry(int i) {
+  constexpr uint64_t kTab[32] = MCX_EXPF_TABLE;
+  return kTab[i];
+}
+
+// expf with the table read from tab (host memory, device global memory or
+// LDS: the caller picks where the 32 entries live).
+MCX_HD inline float mcx_expf_with(float x, const uint64_t *tab) {
   constexpr double kInvLn2N = 0x1.71547652b82fep+0 * 32;  // 32 / ln 2
   constexpr double kShift = 0x1.8p+52;                    // rounds to an integer when added
   constexpr double C0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32;
@@ -68,11 +77,16 @@ MCX_HD inline float mcx_expf(float x) {
   const uint64_t ki = mcx_f64_bits(kd);
   kd -= kShift;
   const double r = __builtin_fma(kInvLn2N, xd, -kd);
-  const double s = mcx_f64_from(kTab[ki & 31u] + (ki << 47));
+  const double s = mcx_f64_from(tab[ki & 31u] + (ki << 47));
   const double z = __builtin_fma(C0, r, C1);
   const double r2 = r * r;
   double y = __builtin_fma(C2, r, 1.0);
   y = __builtin_fma(z, r2, y);
   y = y * s;
   return (float)y;
+}
+
+MCX_HD inline float mcx_expf(float x) {
+  constexpr uint64_t kTab[32] = MCX_EXPF_TABLE;
+  return mcx_expf_with(x, kTab);
 }

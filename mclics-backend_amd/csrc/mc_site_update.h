@@ -47,11 +47,12 @@ __device__ __forceinline__ float sel(const float4 &v, int e) {
 // move to the exact path; below -150 it is 0 and u > 0 rejects, as
 // expf(x) = 0 does; in between |x| < 104 keeps rel far below 1.
 __device__ __forceinline__ int fast_decide(float y, float u, float relc) {
-  if (u == 0.0f) return -1;
   const float rel = fmaf(fabsf(y), 6.60973460e-7f, relc);  // + |x| 2^-20 = |y| ln2 2^-20
   const float p = __builtin_amdgcn_exp2f(y);
-  if (fabsf(u - p) <= p * rel) return -1;
-  return u < p ? 1 : 0;
+  // Branch free (bitwise |), so the two sites of a pair interleave: a zero
+  // uniform is always decided exactly.
+  const bool band = (fabsf(u - p) <= p * rel) | (u == 0.0f);
+  return band ? -1 : (u < p ? 1 : 0);
 }
 
 struct Site {
@@ -145,6 +146,7 @@ struct PassConst {
   float relc;       // RN(eps/T) + 2^-19 for the FP32 screen
   McGhrlConst ghrl;  // GHRL screen constants
   float hs, hb;     // ang_var 2^-24 and -0.5 ang_var (propose)
+  const uint64_t *etab;  // mcx_expf's table for the exact path (global memory unless a kernel points it at LDS)
 };
 
 // Bound on one bond's magnitude from each term's coefficient (unit vectors):
@@ -178,6 +180,7 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
   k.relc = k.eps / k.T + 1.9073486e-6f;
   k.hs = k.av * 5.96046448e-8f;
   k.hb = -0.5f * k.av;
+  k.etab = mcd_exp_table;
   return k;
 }
 
@@ -191,9 +194,7 @@ __device__ __forceinline__ PassConst pass_const(const MCDevParams &P, const MCDe
 // eps = 2^-14 * 1.5|A| = 1024u * 1.5|A|.
 // An absent neighbour is a zero vector here, whose bond term is exactly 0;
 // a surface site is always decided exactly.
-template <bool GEO>
-__device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
-  if (GEO && w.surf >= 0) return -1;
+__device__ __forceinline__ float screen_ll_sum(const Site &S) {
   const float d0 = S.q[0] - S.o[0], d1 = S.q[1] - S.o[1], d2 = S.q[2] - S.o[2];
   const float s0 = S.q[0] + S.o[0], s1 = S.q[1] + S.o[1], s2 = S.q[2] + S.o[2];
   float acc = 0.0f;
@@ -203,12 +204,11 @@ __device__ __forceinline__ int screen_ll(const MCDevParams &P, const PassConst &
     const float e2 = fmaf(s2, S.m[b][2], fmaf(s1, S.m[b][1], s0 * S.m[b][0]));
     acc = fmaf(e1, e2, acc);
   }
-  float y = acc * k.kyLL;
-  if (P.elecA != 0.0f) {
-    const float fd = mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
-    y = fmaf(acc, k.kyLL, fd * k.ky1);
-  }
-  return fast_decide(y, S.u, k.relc);
+  return acc;
+}
+
+__device__ __forceinline__ float field_delta(const MCDevParams &P, const PassConst &k, const Site &S) {
+  return mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
 }
 
 // FP32 screen of one pear move: each bond of the reference
@@ -237,32 +237,64 @@ __device__ __forceinline__ float bond32_pear(const MCDevParams &P, int b, const 
 }
 
 template <bool GEO>
-__device__ __forceinline__ int screen_bonds(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
-  if (GEO && w.surf >= 0) return -1;
+__device__ __forceinline__ float screen_pear_sum(const MCDevParams &P, const Site &S, const Where &w) {
   float acc = 0.0f;
 #pragma unroll
   for (int b = 0; b < 6; ++b) {
     if (GEO && !((w.on >> b) & 1u)) continue;
     acc += bond32_pear(P, b, S.q, S.m[b]) - bond32_pear(P, b, S.o, S.m[b]);
   }
-  float dE = acc;
-  if (P.elecA != 0.0f)
-    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
-  return fast_decide(dE * k.ky1, S.u, k.relc);
+  return acc;
 }
 
-// FP32 screen of one GHRL move: mcx_ghrl_delta (mc_ghrl_screen.h), whose
-// distance from the reference's dE is at most 920u M (+ 6u F with the
-// field) against eps = 2^-14 (2M + F) = 2048u M + 1024u F, over twice that.
-template <bool GEO>
-__device__ __forceinline__ int screen_ghrl(const MCDevParams &P, const PassConst &k, const Site &S, const Where &w) {
-  if (GEO && w.surf >= 0) return -1;
-  bool tiny;
-  float dE = mcx_ghrl_delta(k.ghrl, S.o, S.q, S.m, GEO ? w.on : 0x3Fu, &tiny);
-  if (tiny) return -1;
-  if (P.elecA != 0.0f)
-    dE += mcd_field(P, k.elecE, S.q[0], S.q[1], S.q[2]) - mcd_field(P, k.elecE, S.o[0], S.o[1], S.o[2]);
-  return fast_decide(dE * k.ky1, S.u, k.relc);
+// Screens both sites of a pair (d = 1 accept, 0 reject, -1 decide exactly).
+// Their arithmetic is straight-line code for both sites, so the compiler
+// interleaves the two independent chains; the field is one branch on a
+// uniform value for both.  Surface sites are always decided exactly.
+//   LL:   y = acc kyLL (+ fd ky1 with the field), acc the bond sum above.
+//   GHRL: dE = mcx_ghrl_delta (mc_ghrl_screen.h), within 920u M (+ 6u F with
+//         the field) of the reference's dE against eps = 2^-14 (2M + F) =
+//         2048u M + 1024u F, over twice that; a tiny denominator there sends
+//         the site to the exact path.
+//   pear: dE = the FP32 bond sum above (+ fd).
+template <int POT, bool GEO>
+__device__ __forceinline__ void screen_pair(const MCDevParams &P, const PassConst &k, const Site S[2], const Where &w,
+                                            int &d0, int &d1) {
+  if (GEO && w.surf >= 0) {
+    d0 = d1 = -1;
+    return;
+  }
+  float y0, y1;
+  bool ex0 = false, ex1 = false;
+  if (POT == MCD_LL) {
+    const float a0 = screen_ll_sum(S[0]), a1 = screen_ll_sum(S[1]);
+    if (P.elecA != 0.0f) {
+      y0 = fmaf(a0, k.kyLL, field_delta(P, k, S[0]) * k.ky1);
+      y1 = fmaf(a1, k.kyLL, field_delta(P, k, S[1]) * k.ky1);
+    } else {
+      y0 = a0 * k.kyLL;
+      y1 = a1 * k.kyLL;
+    }
+  } else {
+    float e0, e1;
+    if (POT == MCD_GHRL) {
+      e0 = mcx_ghrl_delta(k.ghrl, S[0].o, S[0].q, S[0].m, GEO ? w.on : 0x3Fu, &ex0);
+      e1 = mcx_ghrl_delta(k.ghrl, S[1].o, S[1].q, S[1].m, GEO ? w.on : 0x3Fu, &ex1);
+    } else {
+      e0 = screen_pear_sum<GEO>(P, S[0], w);
+      e1 = screen_pear_sum<GEO>(P, S[1], w);
+    }
+    if (P.elecA != 0.0f) {
+      e0 += field_delta(P, k, S[0]);
+      e1 += field_delta(P, k, S[1]);
+    }
+    y0 = e0 * k.ky1;
+    y1 = e1 * k.ky1;
+  }
+  d0 = fast_decide(y0, S[0].u, k.relc);
+  d1 = fast_decide(y1, S[1].u, k.relc);
+  if (ex0) d0 = -1;
+  if (ex1) d1 = -1;
 }
 
 // The neighbours of the pair: yp..zm hold, per component, the +y/-y/+z/-z
@@ -305,16 +337,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
   // Screen both sites, then one shared (rarely executed) exact evaluation for
   // whichever sites fell inside the error band.
   int d0, d1;
-  if (POT == MCD_LL) {
-    d0 = screen_ll<GEO>(P, k, S[0], w);
-    d1 = screen_ll<GEO>(P, k, S[1], w);
-  } else if (POT == MCD_GHRL) {
-    d0 = screen_ghrl<GEO>(P, k, S[0], w);
-    d1 = screen_ghrl<GEO>(P, k, S[1], w);
-  } else {
-    d0 = screen_bonds<GEO>(P, k, S[0], w);
-    d1 = screen_bonds<GEO>(P, k, S[1], w);
-  }
+  screen_pair<POT, GEO>(P, k, S, w, d0, d1);
   a[0] = d0 == 1;
   a[1] = d1 == 1;
   if (d0 < 0 || d1 < 0) {
@@ -324,7 +347,7 @@ __device__ __forceinline__ void pair_update(const MCDevParams &P, const PassCons
       const Site &X = t ? S[1] : S[0];
       float Eo, En;
       exact_energies<POT, GEO>(P, k.cA, k.elecE, X, w, Eo, En);
-      const bool at = mcd_accept(En, Eo, k.T, X.u);
+      const bool at = mcd_accept_with(En, Eo, k.T, X.u, k.etab);
       if (t) a[1] = at; else a[0] = at;
     }
   }

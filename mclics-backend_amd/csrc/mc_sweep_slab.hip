@@ -244,6 +244,13 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
 
   float4 P0[3], P1[3], P2[3], P3[3], P4[3], Q[3];
   uint32_t acc = 0;
+  // mcx_expf's table in LDS: the rare exact decisions then read it without
+  // waiting for the vector memory operations in flight (the prefetched
+  // plane, the stores).  The prologue's first barrier publishes it.
+  __shared__ uint64_t s_etab[32];
+  if (threadIdx.x < 32) s_etab[threadIdx.x] = mcx_expf_table_entry((int)threadIdx.x);
+  mcsu::PassConst kt = kc;
+  kt.etab = s_etab;
 
   // Prologue: planes z0-2 .. z0+2; red updates of z0-1 and z0.
   if (active) {
@@ -258,8 +265,8 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   __syncthreads();
   if (active && red_row) {
     if (has_plane<GEO>(g, z0 - 1))
-      update<POT, false, GEO>(P, kc, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
-    acc += update<POT, RECORD, GEO>(P, kc, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
+      update<POT, false, GEO>(P, kt, g, L, 0, P0, Q, P1, Bbuf((z0 - 1) & 1), y, z0 - 1, false, rec);
+    acc += update<POT, RECORD, GEO>(P, kt, g, L, 0, P1, P0, P2, Bbuf(z0 & 1), y, z0, owned_row, rec);
     put_half(Rbuf(z0 & 1), L, ((y + z0) & 1), P1);
   }
   __syncthreads();
@@ -276,12 +283,12 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
       load_seg<GEO>(in, g, L, it <= CZ - 2 ? p + 3 : p + 2, P4);
       if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
       if (red_row && has_plane<GEO>(g, p + 1)) {
-        acc += update<POT, RECORD, GEO>(P, kc, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
+        acc += update<POT, RECORD, GEO>(P, kt, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
                                    owned_row && it <= CZ - 2, rec);
         if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);
       }
       if (owned_row) {
-        acc += update<POT, RECORD, GEO>(P, kc, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
+        acc += update<POT, RECORD, GEO>(P, kt, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
         const int off = g.XY * p + L.rowbase;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {  // streaming stores: nothing reads them before the next launch
@@ -701,7 +708,7 @@ bool mc_slab_shape_ok(const MCDevParams &P, const MCSlabShape &s, size_t lds_lim
   if (s.ty < 1 || s.cz < 1 || s.ty > P.ny || s.cz > P.nz) return false;
   if (P.ny % s.ty || P.nz % s.cz) return false;
   if ((P.nx / 4) * (s.ty + 4) > kMaxThreads) return false;
-  return mc_slab_lds_bytes(P, s.ty) + 256 <= lds_limit;  // + the static reduction array
+  return mc_slab_lds_bytes(P, s.ty) + 512 <= lds_limit;  // + the static arrays (reduction, exp table)
 }
 
 size_t mc_measure_slab_lds_bytes(const MCDevParams &P, int ty) {
