@@ -123,7 +123,7 @@ def chain_stats(lat, sweep_us, budget_s, target=0.2, min_sweeps=20000):
 
 
 def learned_leg(budget_s=30.0, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.14), epochs=40, schedule=None,
-                target=0.2):
+                target=0.2, reflect=True):
     """Config C5: 128^3 LL near the transition.  The network is trained on
     n_train neighbourhoods drawn from plain-sampler lattices at the three
     temperatures, then plain and learned chains start from the same
@@ -135,6 +135,7 @@ def learned_leg(budget_s=30.0, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
     import mclics
     nt = L5**3
     out = {"workload": f"LL bulk {L5}^3 periodic, A=1, T in {list(temps)}", "n_train": n_train,
+           "moves": "vMF draw, then reflection through the learned mean" if reflect else "vMF draw only",
            "budget_s_per_chain": budget_s, "tau_rel_err_target": target, "per_T": []}
     starts = {}
     with mclics.Network(seed=5) as net:
@@ -162,7 +163,7 @@ def learned_leg(budget_s=30.0, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
             for method in ("plain", "learned"):
                 with mclics.Lattice(mclics.Config(L5, L5, L5, T=T, seed=90 + ti), starts[T]) as lat:
                     if method == "learned":
-                        lat.set_learned(net)
+                        lat.set_learned(net, reflect=reflect)
                         lat.sweeps(200)
                         ms, _ = lat.time_sweeps(300, graph=False)
                         sps = 300 / (ms / 1e3)
@@ -369,6 +370,48 @@ def copy_probe(nbytes=201326592, reps=20):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def stream_ceiling(nbytes=3 * 4 * 256**3, reps=100):
+    """Ping-pong device copies of one 256^3 plane set (A -> B, B -> A, ...):
+    every copy reads what the previous one wrote, as consecutive sweeps do,
+    so neither side is left in the 256 MB last-level cache from a repeat of
+    the same input.  Gives the read+write rate the HBM sustains for the
+    sweep's bytes without any update work."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 0, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(b, 0, C.c_size_t(nbytes)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    d2d = 3  # hipMemcpyDeviceToDevice
+
+    def pingpong(n):
+        for i in range(n):
+            src, dst = (a, b) if i % 2 == 0 else (b, a)
+            assert hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), d2d, None) == 0
+
+    pingpong(20)
+    assert hip.hipEventRecord(e0, None) == 0
+    pingpong(reps)
+    assert hip.hipEventRecord(e1, None) == 0
+    assert hip.hipEventSynchronize(e1) == 0
+    ms = C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    us = ms.value * 1e3 / reps
+    tbs = 2.0 * nbytes / (us * 1e-6) / 1e12
+    return {"method": "hipMemcpyAsync device-to-device ping-pong of one 256^3 plane set (3 x 67.1 MB)",
+            "copies": reps, "us_per_copy": us, "read_plus_write_TB_s": tbs,
+            "frac_of_8TBs": tbs / (HBM_PEAK_GBS / 1e3),
+            "sweep_floor_us": us, "note": "a sweep moves the same bytes, so this copy time is the floor a "
+                                          "sweep could reach with every update hidden under memory traffic"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +428,8 @@ def main():
                          "tau_int(S) is known to --learned-target)")
     ap.add_argument("--learned-target", type=float, default=0.2,
                     help="relative error of tau_int(S) at which a chain of the learned-proposal leg stops")
+    ap.add_argument("--learned-no-reflect", action="store_true",
+                    help="learned sweeps make the vMF move only (no reflection through the learned mean)")
     ap.add_argument("--kernel", choices=("auto", "slab", "x4", "generic"), default="auto",
                     help="sweep kernel of the headline leg (auto: slab where it fits)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -487,6 +532,7 @@ def main():
                                                         "of back-to-back launches",
                      "event_ms": region_ms,
                      "slab_tile": tile},
+        "stream_ceiling": stream_ceiling(),
         "observables": obs,
         "measured_sweeps": measured_step,
         "last_acceptance": acc / nt,
@@ -502,7 +548,7 @@ def main():
     if not args.no_learned:
         log("bench: learned-proposal leg (128^3, T near T_NI)")
         result["learned_proposal"] = learned_leg(args.learned_budget, epochs=args.learned_epochs,
-                                                 target=args.learned_target)
+                                                 target=args.learned_target, reflect=not args.learned_no_reflect)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         result["cpu_baseline"] = cpu_baseline(n0)

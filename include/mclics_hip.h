@@ -307,10 +307,18 @@ int mc_mlp_dataset_clear(mc_mlp *m);
 int mc_mlp_train(mc_mlp *m, int epochs, int batch, float lr, float *loss_per_epoch);
 /* Attach (every > 0: every k-th sweep uses the network) or detach (NULL or 0). */
 int mc_set_learned_proposal(mc_handle *h, mc_mlp *net, int every);
+/* Learned sweeps follow each site's vMF move with a second Metropolis-
+ * Hastings move, the reflection of the director through the network's mean
+ * direction mu (a symmetric proposal; on by default).  on = 0 leaves the
+ * vMF move alone. */
+int mc_set_learned_reflection(mc_handle *h, int on);
 /* One learned sweep recording per site mu (AoS), log kappa, the proposal
- * (AoS), the accept uniform, the decision and the FP32 energy change. */
+ * (AoS), the accept uniform, the decision and the FP32 energy change of the
+ * vMF move, then the reflected director (AoS), its accept uniform, decision
+ * and FP32 energy change (zeros when the reflection is off).  Any output may
+ * be NULL. */
 int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_aos, float *u, uint8_t *accept,
-                            float *dE);
+                            float *dE, float *refl_aos, float *u2, uint8_t *accept2, float *dE2);
 
 #ifdef __cplusplus
 }

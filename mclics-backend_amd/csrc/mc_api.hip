@@ -81,6 +81,9 @@ struct mc_handle {
   int x4_xcd = 0;               // tile schedule of the segment pass
   mc_mlp *learned = nullptr;    // learned proposal network (null: plain sweeps)
   int learned_every = 0;        // every k-th sweep uses the learned proposal
+  int learned_reflect = 1;      // learned sweeps add the reflection move
+  float *lrbuf = nullptr;       // learned-sweep record of the reflection move (5 planes)
+  uint8_t *lrabuf = nullptr;    // and its decisions
   long long host_sweeps = 0;    // sweeps enqueued so far (selects learned sweeps)
   int cu_count = 256;
   float *nx = nullptr, *ny = nullptr, *nz = nullptr;  // current lattice
@@ -138,7 +141,7 @@ void free_handle(mc_handle *h) {
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   void *ptrs[] = {h->nx, h->ny, h->nz, h->bx, h->by, h->bz, h->pt, h->sx, h->sy, h->sz, h->st, h->ctr, h->partials,
-                  h->result, h->staging, h->pbuf, h->abuf, h->series};
+                  h->result, h->staging, h->pbuf, h->abuf, h->series, h->lrbuf, h->lrabuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -252,7 +255,7 @@ void launch_pass(mc_handle *h, int t, const MCRecord *rec, const MCLearnRecord *
     // one grid for both colours (the sweep counters need it)
     const int slots = std::max(mcd_set_slots(h->P, 8), mcd_set_slots(h->P, 9));
     const int grid = mc_mlp_learned_grid(h->P, slots, h->cu_count);
-    mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, lrec);
+    mc_mlp_launch_learned_pass(L, mc_mlp_buffers(h->learned), t, grid, finalize, h->learned_reflect, lrec);
   } else if (use_slab(h)) {
     mc_launch_sweep_slab(L, h->bx, h->by, h->bz, slab_shape(h), rec);
     swap_planes(h);
@@ -1174,14 +1177,25 @@ int mc_set_learned_proposal(mc_handle *h, mc_mlp *net, int every) {
   return MC_OK;
 }
 
+int mc_set_learned_reflection(mc_handle *h, int on) {
+  int rc = check_handle(h);
+  if (rc) return rc;
+  if (on != 0 && on != 1) return fail(MC_EINVAL, "mc_set_learned_reflection: on must be 0 or 1");
+  MC_HIP(hipStreamSynchronize(h->stream));
+  h->learned_reflect = on;
+  return MC_OK;
+}
+
 int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_aos, float *u, uint8_t *accept,
-                            float *dE) {
+                            float *dE, float *refl_aos, float *u2, uint8_t *accept2, float *dE2) {
   int rc = check_handle(h);
   if (rc) return rc;
   if (!h->learned) return fail(MC_ESTATE, "mc_record_learned_sweep: no learned proposal attached");
   rc = ensure_parity_buffers(h);
   if (rc) return rc;
   const size_t nt = (size_t)h->P.nt;
+  if (!h->lrbuf) MC_HIP(hipMalloc(&h->lrbuf, 5 * nt * sizeof(float)));
+  if (!h->lrabuf) MC_HIP(hipMalloc(&h->lrabuf, nt));
   MCLearnRecord rec;
   rec.mu = h->staging;  // 3 planes, AoS
   rec.prop = h->pbuf;   // 3 planes, AoS
@@ -1189,9 +1203,15 @@ int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_ao
   rec.u = h->pbuf + 4 * nt;
   rec.dE = h->pbuf + 5 * nt;
   rec.acc = h->abuf;
+  rec.refl = h->lrbuf;  // 3 planes, AoS
+  rec.u2 = h->lrbuf + 3 * nt;
+  rec.dE2 = h->lrbuf + 4 * nt;
+  rec.acc2 = h->lrabuf;
   MC_HIP(hipMemsetAsync(h->pbuf, 0, 6 * nt * sizeof(float), h->stream));
   MC_HIP(hipMemsetAsync(h->staging, 0, 3 * nt * sizeof(float), h->stream));
   MC_HIP(hipMemsetAsync(h->abuf, 0, nt, h->stream));
+  MC_HIP(hipMemsetAsync(h->lrbuf, 0, 5 * nt * sizeof(float), h->stream));
+  MC_HIP(hipMemsetAsync(h->lrabuf, 0, nt, h->stream));
   const long long keep = h->host_sweeps;
   h->host_sweeps = 0;  // force a learned sweep
   const int every = h->learned_every;
@@ -1206,6 +1226,10 @@ int mc_record_learned_sweep(mc_handle *h, float *mu, float *logk, float *prop_ao
   if (u) MC_HIP(hipMemcpyAsync(u, rec.u, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   if (dE) MC_HIP(hipMemcpyAsync(dE, rec.dE, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   if (accept) MC_HIP(hipMemcpyAsync(accept, rec.acc, nt, hipMemcpyDeviceToHost, h->stream));
+  if (refl_aos) MC_HIP(hipMemcpyAsync(refl_aos, rec.refl, 3 * nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (u2) MC_HIP(hipMemcpyAsync(u2, rec.u2, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (dE2) MC_HIP(hipMemcpyAsync(dE2, rec.dE2, nt * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  if (accept2) MC_HIP(hipMemcpyAsync(accept2, rec.acc2, nt, hipMemcpyDeviceToHost, h->stream));
   MC_HIP(hipStreamSynchronize(h->stream));
   return MC_OK;
 }

@@ -30,6 +30,11 @@ struct MCLearnRecord {
   float *u;     // [Nt]
   uint8_t *acc; // [Nt]
   float *dE;    // [Nt]
+  // second move (reflection through mu), recorded when refl is non-null
+  float *refl;   // [3*Nt] reflected director
+  float *u2;     // [Nt] its accept uniform
+  uint8_t *acc2; // [Nt]
+  float *dE2;    // [Nt] its FP32 energy change
 };
 
 void mc_mlp_launch_pack(const MCMLPBuffers &B, hipStream_t s);
@@ -43,8 +48,9 @@ void mc_mlp_launch_dataset(const MCLaunch &L, int n, uint32_t seed, float *X, fl
 // Workgroups of a learned pass over `slots` sites: one 64-site block per
 // wave, up to the workgroups every CU holds at once.
 int mc_mlp_learned_grid(const MCDevParams &P, int slots, int cu_count);
+// reflect: follow each vMF move with the reflection through mu (mc_mlp.hip).
 void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
-                                const MCLearnRecord *rec);
+                                int reflect, const MCLearnRecord *rec);
 
 struct mc_mlp;
 struct mc_handle;

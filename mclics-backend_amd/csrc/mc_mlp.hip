@@ -18,6 +18,13 @@
 // Because mu and kappa depend on the neighbours only, which a pass never
 // changes at the updated site, the Metropolis-Hastings ratio of a move
 // o -> n is exp(-dE/T) cosh(kappa mu.o) / cosh(kappa mu.n).
+// A second move follows (unless switched off): the reflection of the
+// site's director c through mu, r = 2 (mu.c) mu - c.  For a fixed mu it is
+// an involution with unit Jacobian, a symmetric proposal, so its ratio is
+// exp(-dE/T) alone.  Where mu is the principal axis of the site's local
+// field (for Lebwohl-Lasher, of sum_b m_b m_b^T) the reflection conserves
+// the energy exactly: an over-relaxation move, accepted almost always and
+// far from the current director, driven by the network's mean direction.
 #include "mc_kernels.h"
 #include "mc_mlp.h"
 
@@ -155,27 +162,43 @@ __device__ __forceinline__ void forward64l(const __bf16 *Wl, const float *Bl, co
   const int r = l & 15, g = l >> 4;
   const bf16x8 *W = reinterpret_cast<const bf16x8 *>(Wl) + l;  // operand b at W[64 b]
 #pragma unroll 1
-  for (int t = 0; t < 4; ++t) {
-    // Compiler-only barrier: the operands are read again for every tile
-    // instead of being hoisted out of the loop into 88 long-lived registers.
+  for (int t = 0; t < 4; t += 2) {
+    // Two 16-site tiles per trip: their MFMA chains are independent, so the
+    // second tile's products issue while the first tile's results are in
+    // flight.  Compiler-only barrier: the operands are read again for every
+    // trip instead of being hoisted out of the loop into long-lived registers.
     asm volatile("" ::: "memory");
-    const bf16x8 x = g < 3 ? load8(Xs + (16 * t + r) * 24 + 8 * g) : bf16x8{};
-    f4 h1[4];
+    bf16x8 x[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) x[u] = g < 3 ? load8(Xs + (16 * (t + u) + r) * 24 + 8 * g) : bf16x8{};
+    f4 h1[2][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
-      h1[mt] = mfma(W[64 * mt], x, *reinterpret_cast<const f4 *>(Bl + 16 * mt + 4 * g));
-    const bf16x8 a0 = pack_hidden(h1[0], h1[1]), a1 = pack_hidden(h1[2], h1[3]);
-    f4 h2[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      f4 c = mfma(W[64 * (4 + 2 * mt)], a0, *reinterpret_cast<const f4 *>(Bl + 64 + 16 * mt + 4 * g));
-      h2[mt] = mfma(W[64 * (5 + 2 * mt)], a1, c);
+      for (int u = 0; u < 2; ++u)
+        h1[u][mt] = mfma(W[64 * mt], x[u], *reinterpret_cast<const f4 *>(Bl + 16 * mt + 4 * g));
+    bf16x8 a0[2], a1[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      a0[u] = pack_hidden(h1[u][0], h1[u][1]);
+      a1[u] = pack_hidden(h1[u][2], h1[u][3]);
     }
-    const bf16x8 b0 = pack_hidden(h2[0], h2[1]), b1 = pack_hidden(h2[2], h2[3]);
-    f4 o = g == 0 ? *reinterpret_cast<const f4 *>(Bl + 128) : f4{0.0f, 0.0f, 0.0f, 0.0f};
-    o = mfma(W[64 * 12], b0, o);
-    o = mfma(W[64 * 13], b1, o);
-    if (g == 0) *reinterpret_cast<f4 *>(Os + (16 * t + r) * 4) = o;
+    f4 h2[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        f4 c = mfma(W[64 * (4 + 2 * mt)], a0[u], *reinterpret_cast<const f4 *>(Bl + 64 + 16 * mt + 4 * g));
+        h2[u][mt] = mfma(W[64 * (5 + 2 * mt)], a1[u], c);
+      }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const bf16x8 b0 = pack_hidden(h2[u][0], h2[u][1]), b1 = pack_hidden(h2[u][2], h2[u][3]);
+      f4 o = g == 0 ? *reinterpret_cast<const f4 *>(Bl + 128) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+      o = mfma(W[64 * 12], b0, o);
+      o = mfma(W[64 * 13], b1, o);
+      if (g == 0) *reinterpret_cast<f4 *>(Os + (16 * (t + u) + r) * 4) = o;
+    }
   }
   wave_sync();
 }
@@ -715,7 +738,7 @@ constexpr int kLearnWaves = 4;
 template <int POT, bool RECORD, bool PER>
 __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_learned_pass(
     MCDevParams P, const uint4 *__restrict__ ops, float *__restrict__ nx, float *__restrict__ ny,
-    float *__restrict__ nz, MCDevState *st, int colour, int finalize, MCLearnRecord rec) {
+    float *__restrict__ nz, MCDevState *st, int colour, int finalize, int reflect, MCLearnRecord rec) {
   __shared__ __align__(16) uint4 Wst[MC_MLP_OPS_BYTES / 16];
   __shared__ __align__(16) __bf16 Xall[kLearnWaves][64 * 24];
   __shared__ __align__(16) float Oall[kLearnWaves][64 * 4];
@@ -788,10 +811,55 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     const float xlog = -dE * invT + lr;
     const float u = mcd_u01(r.w);
     const bool a = valid && u < __expf(fminf(xlog, 80.0f));
-    if (a) {
-      nx[s] = q[0];
-      ny[s] = q[1];
-      nz[s] = q[2];
+    // The reflection move from the director c the first move left.  Its
+    // accept uniform is the top 24 bits of word z (whose bit 0 chose the
+    // vMF mixture sign).  The reflected vector is renormalised to first
+    // order (|r| = 1 up to a few ulps already).
+    float c0 = a ? q[0] : o0, c1 = a ? q[1] : o1, c2 = a ? q[2] : o2;
+    float r0 = c0, r1 = c1, r2 = c2, dE2 = 0.0f, u2 = 0.0f;
+    bool a2 = false;
+    if (reflect) {
+      const float dm = mu[0] * c0 + mu[1] * c1 + mu[2] * c2;
+      r0 = 2.0f * dm * mu[0] - c0;
+      r1 = 2.0f * dm * mu[1] - c1;
+      r2 = 2.0f * dm * mu[2] - c2;
+      const float rn = fmaf(-0.5f, fmaf(r2, r2, fmaf(r1, r1, r0 * r0)), 1.5f);
+      r0 *= rn;
+      r1 *= rn;
+      r2 *= rn;
+      if (POT == MCD_LL) {
+        float b2 = 0.0f;
+#pragma unroll
+        for (int bb = 0; bb < 6; ++bb) {
+          if (!on[bb]) continue;
+          const float ac = fmaf(c2, m[bb][2], fmaf(c1, m[bb][1], c0 * m[bb][0]));
+          const float ar = fmaf(r2, m[bb][2], fmaf(r1, m[bb][1], r0 * m[bb][0]));
+          b2 = fmaf(ar - ac, ar + ac, b2);
+        }
+        dE2 = cAf * b2;
+      } else {
+        double so = 0.0, sn = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 6; ++bb) {
+          if (!on[bb]) continue;
+          so += (double)mcd_bond<POT>(P, cA, bb, c0, c1, c2, m[bb][0], m[bb][1], m[bb][2]);
+          sn += (double)mcd_bond<POT>(P, cA, bb, r0, r1, r2, m[bb][0], m[bb][1], m[bb][2]);
+        }
+        dE2 = (float)sn - (float)so;
+      }
+      if (P.elecA != 0.0f) dE2 += mcd_field(P, E, r0, r1, r2) - mcd_field(P, E, c0, c1, c2);
+      u2 = mcd_u01(r.z);
+      a2 = valid && u2 < __expf(fminf(-dE2 * invT, 80.0f));
+    }
+    if (a2) {
+      c0 = r0;
+      c1 = r1;
+      c2 = r2;
+    }
+    if (a || a2) {
+      nx[s] = c0;
+      ny[s] = c1;
+      nz[s] = c2;
     }
     acc += a ? 1u : 0u;
     if (RECORD && valid) {
@@ -805,6 +873,14 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
       rec.u[s] = u;
       rec.acc[s] = a ? 1 : 0;
       rec.dE[s] = dE;
+      if (rec.refl && reflect) {
+        rec.refl[3 * (size_t)s] = r0;
+        rec.refl[3 * (size_t)s + 1] = r1;
+        rec.refl[3 * (size_t)s + 2] = r2;
+        rec.u2[s] = u2;
+        rec.acc2[s] = a2 ? 1 : 0;
+        rec.dE2[s] = dE2;
+      }
     }
     wave_sync();  // this wave's Os reads before the next block's Xs/Os writes
   }
@@ -905,14 +981,14 @@ int mc_mlp_learned_grid(const MCDevParams &P, int slots, int cu_count) {
 }
 
 void mc_mlp_launch_learned_pass(const MCLaunch &L, const MCMLPBuffers &B, int colour, int grid, int finalize,
-                                const MCLearnRecord *rec) {
+                                int reflect, const MCLearnRecord *rec) {
   const MCLearnRecord none{};
   const MCLearnRecord &R = rec ? *rec : none;
   const uint4 *ops = reinterpret_cast<const uint4 *>(B.ops);
   const int threads = 64 * kLearnWaves;
   const bool per = L.P.per[0] && L.P.per[1] && L.P.per[2];
 #define MC_LEARNED_(POT_, REC_, PER_) \
-  k_learned_pass<POT_, REC_, PER_><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, R)
+  k_learned_pass<POT_, REC_, PER_><<<grid, threads, 0, L.stream>>>(L.P, ops, L.nx, L.ny, L.nz, L.st, colour, finalize, reflect, R)
 #define MC_LEARNED(POT_)                  \
   if (rec) {                              \
     if (per) MC_LEARNED_(POT_, true, true);   \

@@ -91,6 +91,7 @@ EXPORTS = (
     "mc_mlp_dataset_clear",
     "mc_mlp_train",
     "mc_set_learned_proposal",
+    "mc_set_learned_reflection",
     "mc_record_learned_sweep",
 )
 
@@ -212,7 +213,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.mc_mlp_dataset_clear.argtypes = [vp]
     lib.mc_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, f32p]
     lib.mc_set_learned_proposal.argtypes = [vp, vp, C.c_int]
-    lib.mc_record_learned_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p]
+    lib.mc_set_learned_reflection.argtypes = [vp, C.c_int]
+    lib.mc_record_learned_sweep.argtypes = [vp, f32p, f32p, f32p, f32p, u8p, f32p, f32p, f32p, u8p, f32p]
     for name in EXPORTS:
         if name not in ("mc_last_error", "mc_device_count"):
             getattr(lib, name).restype = C.c_int
@@ -445,17 +447,24 @@ class Lattice:
         _check(self._lib.mc_pass(self._h, set_id, _f32(prop), _f32(uu), _u8(acc), _f32(eo), _f32(en)))
         return acc, eo, en
 
-    def set_learned(self, net: Optional["Network"], every: int = 1) -> None:
+    def set_learned(self, net: Optional["Network"], every: int = 1, reflect: bool = True) -> None:
+        """Attach a network (every k-th sweep learned) or detach it (None).
+        reflect: each vMF move is followed by the reflection through the
+        network's mean direction (a second Metropolis-Hastings move)."""
         _check(self._lib.mc_set_learned_proposal(self._h, None if net is None else net._h, 0 if net is None else every))
+        _check(self._lib.mc_set_learned_reflection(self._h, 1 if reflect else 0))
         self._net = net
 
     def record_learned_sweep(self):
         nt = self.nt
         r = {"mu": np.zeros(3 * nt, np.float32), "logk": np.zeros(nt, np.float32),
              "prop": np.zeros(3 * nt, np.float32), "u": np.zeros(nt, np.float32),
-             "accept": np.zeros(nt, np.uint8), "dE": np.zeros(nt, np.float32)}
+             "accept": np.zeros(nt, np.uint8), "dE": np.zeros(nt, np.float32),
+             "refl": np.zeros(3 * nt, np.float32), "u2": np.zeros(nt, np.float32),
+             "accept2": np.zeros(nt, np.uint8), "dE2": np.zeros(nt, np.float32)}
         _check(self._lib.mc_record_learned_sweep(self._h, _f32(r["mu"]), _f32(r["logk"]), _f32(r["prop"]),
-                                                 _f32(r["u"]), _u8(r["accept"]), _f32(r["dE"])))
+                                                 _f32(r["u"]), _u8(r["accept"]), _f32(r["dE"]), _f32(r["refl"]),
+                                                 _f32(r["u2"]), _u8(r["accept2"]), _f32(r["dE2"])))
         return r
 
     def record_sweep(self):

@@ -128,14 +128,19 @@ def mh_window(dE_T, kap, lk, co, cn, x):
     return 2.0 * (2 * abs(dE_T) * U32 + zerr + lc + U32 * (zo + zn) + (2 * abs(x) + 3) * U32)
 
 
-@pytest.mark.parametrize("dims,bound", [((12, 12, 12), (1, 1, 1)), ((7, 6, 6), (0, 1, 1))])
-def test_learned_sweep_decisions_are_exact_mh(dims, bound):
+@pytest.mark.parametrize("dims,bound,reflect", [((12, 12, 12), (1, 1, 1), True), ((7, 6, 6), (0, 1, 1), True),
+                                                ((12, 12, 12), (1, 1, 1), False)])
+def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect):
     # Replays one recorded learned sweep: the oracle recomputes the energy
     # change from the reference site energies (the recorded FP32 dE must be
     # within the FMA form's proven distance of it) and, in FP64 from the
     # recorded dE, mu, log kappa and proposal, the log acceptance ratio; every
     # move outside its own FP32 error window (mh_window, a few 1e-6 at
     # kappa ~ 1) must be decided the same way.
+    # The second move (reflect) is replayed the same way: the reflected
+    # director must be 2 (mu.c) mu - c of the director c the first move left,
+    # its dE the reference's, and its decision u2 < exp(-dE2/T) outside the
+    # FP32 window of -dE2 RN(1/T) and the hardware exp.
     # The 7 x 6 x 6 lattice with a free x axis has colour-set slots past the
     # last column.
     nx, ny, nz = dims
@@ -148,9 +153,12 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound):
                                                   mclics.random_ic(nx, ny, nz, seed=5)) as lat:
         lat.sweeps(50)
         start = lat.directors()
-        lat.set_learned(net)
+        lat.set_learned(net, reflect=reflect)
         rec = lat.record_learned_sweep()
         end = lat.directors()
+    if not reflect:
+        for key in ("refl", "u2", "accept2", "dE2"):
+            assert not rec[key].any(), key
     state = start.copy().reshape(-1, 3)
     T32 = float(np.float32(T))
     near, checked, widest = 0, 0, 0.0
@@ -187,8 +195,31 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound):
                 acc = u < math.exp(min(x, 700.0))
                 assert acc == bool(rec["accept"][s]), (colour, s, x, u, win)
             checked += 1
+            c = qd if acc else o
             if acc:
                 state[s] = q
+            if not reflect:
+                continue
+            r = rec["refl"][3 * s:3 * s + 3]
+            r_ref = 2.0 * float(mu @ c) * mu - c
+            assert np.abs(r.astype(np.float64) - r_ref).max() < 1e-5, (s, r, r_ref)
+            cf = state[s].copy()
+            ec = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, O.f32(cf))
+            er = O.lib().orc_site_energy(C.byref(po), O.f32(flat), O.i32(pt), None, i, j, k, O.f32(r.copy()))
+            dE2 = float(rec["dE2"][s])
+            assert abs(dE2 - (float(er) - float(ec))) <= dE_tol, (s, dE2, float(er) - float(ec))
+            x2 = -dE2 / T32
+            win2 = 2.0 * (2 * abs(x2) + abs(x2) + 3) * U32
+            u2 = float(rec["u2"][s])
+            if (u2 > 0 and abs(math.log(u2) - x2) <= win2) or (u2 == 0 and x2 < -80):
+                near += 1
+                acc2 = bool(rec["accept2"][s])
+            else:
+                acc2 = u2 < math.exp(min(x2, 700.0))
+                assert acc2 == bool(rec["accept2"][s]), (colour, s, x2, u2, win2)
+            checked += 1
+            if acc2:
+                state[s] = r
         # next colour sees this colour's accepted moves
     print(f"learned MH replay: {checked} moves, {near} inside their window, widest window {widest:.2e}")
     assert widest < 1e-4
