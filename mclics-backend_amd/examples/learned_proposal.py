@@ -2,8 +2,10 @@
 
 Equilibrates a lattice with plain sweeps, trains the 22-64-64-4 proposal
 network on neighbourhoods drawn from it, then runs learned sweeps (exact
-Metropolis-Hastings with the network's axial von Mises-Fisher proposal) and
-prints their rate and acceptance next to plain sweeps of the same lattice.
+Metropolis-Hastings with the network's axial von Mises-Fisher proposal, then
+the reflection of each director through the network's mean direction) and
+prints their rate and acceptance (of the vMF move) next to plain sweeps of the
+same lattice.
 
     python mclics-backend_amd/examples/learned_proposal.py [--sweeps N] [--T T]
 """
