@@ -640,12 +640,14 @@ __device__ __forceinline__ void gather6(const MCDevParams &P, const float *nx, c
     const int idx[6] = {i + 1 == X ? s + 1 - X : s + 1,     i == 0 ? s - 1 + X : s - 1,
                         j + 1 == Y ? s + X - XY : s + X,    j == 0 ? s - X + XY : s - X,
                         k + 1 == Z ? s + XY - P.nt : s + XY, k == 0 ? s - XY + P.nt : s - XY};
+    // 32-bit byte offsets from a uniform base (no 64-bit address per load)
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
       on[b] = true;
-      m[b][0] = nx[idx[b]];
-      m[b][1] = ny[idx[b]];
-      m[b][2] = nz[idx[b]];
+      const uint32_t off = (uint32_t)idx[b] * 4u;
+      m[b][0] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(nx) + off);
+      m[b][1] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(ny) + off);
+      m[b][2] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(nz) + off);
     }
     return;
   }
@@ -705,22 +707,20 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
   float w = 1.0f + __logf(v + (1.0f - v) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
   w = fminf(fmaxf(w, -1.0f), 1.0f);
   const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - w * w));
-  // orthonormal e1, e2 perpendicular to mu
-  const float ax = fabsf(mu[0]), ay = fabsf(mu[1]), az = fabsf(mu[2]);
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  if (ax <= ay && ax <= az) a0 = 1.0f; else if (ay <= az) a1 = 1.0f; else a2 = 1.0f;
-  float e0 = mu[1] * a2 - mu[2] * a1, e1 = mu[2] * a0 - mu[0] * a2, e2 = mu[0] * a1 - mu[1] * a0;
-  const float ie = rsqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-  e0 *= ie;
-  e1 *= ie;
-  e2 *= ie;
-  const float f0 = mu[1] * e2 - mu[2] * e1, f1 = mu[2] * e0 - mu[0] * e2, f2 = mu[0] * e1 - mu[1] * e0;
+  // orthonormal e, f perpendicular to the unit mu, branch free (Duff et al.,
+  // "Building an orthonormal basis, revisited", JCGT 6(1), 2017)
+  const float sgn = mu[2] >= 0.0f ? 1.0f : -1.0f;
+  const float ia = -__builtin_amdgcn_rcpf(sgn + mu[2]);
+  const float bb = mu[0] * mu[1] * ia;
+  const float e0 = 1.0f + sgn * mu[0] * mu[0] * ia, e1 = sgn * bb, e2 = -sgn * mu[0];
+  const float f0 = bb, f1 = sgn + mu[1] * mu[1] * ia, f2 = -mu[1];
   const float ph = u_phi;  // revolutions
   const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
   float q0 = w * mu[0] + st * (cs * e0 + sn * f0);
   float q1 = w * mu[1] + st * (cs * e1 + sn * f1);
   float q2 = w * mu[2] + st * (cs * e2 + sn * f2);
-  const float iq = rsqrtf(q0 * q0 + q1 * q1 + q2 * q2);
+  // |q|^2 = 1 + d with d a few ulps: q (1.5 - 0.5 |q|^2) is unit to second order
+  const float iq = fmaf(-0.5f, fmaf(q2, q2, fmaf(q1, q1, q0 * q0)), 1.5f);
   const float sg = neg ? -iq : iq;
   q[0] = q0 * sg;
   q[1] = q1 * sg;
