@@ -37,6 +37,8 @@
 // mc_site_update.h with the same counters as the segment kernel, so a sweep
 // here leaves the lattice, the acceptance count and ang_var exactly as the
 // segment kernel's two passes do.
+#include <type_traits>
+
 #include "mc_kernels.h"
 #include "mc_site_update.h"
 
@@ -273,37 +275,57 @@ __device__ __forceinline__ uint32_t slab_tile(const MCDevParams &P, const mcsu::
   if (active) put_half(Bbuf((z0 + 1) & 1), L, ((y + z0 + 1 + 1) & 1), P2);
   __syncthreads();
 
-  for (int it = 0; it < CZ; ++it) {
-    const int p = z0 + it;
-    if (active) {
-      // The ring's last slot is loaded in every iteration (plane p+3, and in
-      // the last iteration plane p+2 again, which is not used): a
-      // conditional load would make the compiler re-initialise the slot in
-      // every iteration.
-      load_seg<GEO>(in, g, L, it <= CZ - 2 ? p + 3 : p + 2, P4);
-      if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
-      if (red_row && has_plane<GEO>(g, p + 1)) {
-        acc += update<POT, RECORD, GEO>(P, kt, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
-                                   owned_row && it <= CZ - 2, rec);
-        if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);
-      }
-      if (owned_row) {
-        acc += update<POT, RECORD, GEO>(P, kt, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
-        const int off = g.XY * p + L.rowbase;
+  // The walk.  ROLE 2: an owned row (red and black updates, plane stores);
+  // ROLE 1: a row that is not owned (the red update on red halo rows, no
+  // stores); ROLE 0: the role read from the row.  With one row to a wave
+  // (SR = 64) the owned-row waves and the others run separate copies of the
+  // loop.  Each copy then issues the same memory operations in every
+  // iteration, so the wait for the prefetched plane at the top of an
+  // iteration counts out the plane stores issued after that plane's loads
+  // instead of waiting for them too (vector loads and stores share one
+  // in-order counter).
+  auto walk = [&](auto role) {
+    constexpr int ROLE = decltype(role)::value;
+    const bool red = ROLE == 2 || red_row;
+    const bool own = ROLE == 2 || (ROLE == 0 && owned_row);
+    for (int it = 0; it < CZ; ++it) {
+      const int p = z0 + it;
+      if (active) {
+        // The ring's last slot is loaded in every iteration (plane p+3, and in
+        // the last iteration plane p+2 again, which is not used): a
+        // conditional load would make the compiler re-initialise the slot in
+        // every iteration.
+        load_seg<GEO>(in, g, L, it <= CZ - 2 ? p + 3 : p + 2, P4);
+        if (it <= CZ - 2) put_half(Bbuf(p & 1), L, ((y + p + 2 + 1) & 1), P3);
+        if (red && has_plane<GEO>(g, p + 1)) {
+          acc += update<POT, RECORD, GEO>(P, kt, g, L, 0, P2, P1, P3, Bbuf((p + 1) & 1), y, p + 1,
+                                          own && it <= CZ - 2, rec);
+          if (it <= CZ - 2) put_half(Rbuf((p + 1) & 1), L, ((y + p + 1) & 1), P2);
+        }
+        if (own) {
+          acc += update<POT, RECORD, GEO>(P, kt, g, L, 1, P1, P0, P2, Rbuf(p & 1), y, p, true, rec);
+          const int off = g.XY * p + L.rowbase;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {  // streaming stores: nothing reads them before the next launch
-          typedef float nf4 __attribute__((ext_vector_type(4)));
-          const nf4 v = {P1[d].x, P1[d].y, P1[d].z, P1[d].w};
-          __builtin_nontemporal_store(v, reinterpret_cast<nf4 *>(out[d] + off));
+          for (int d = 0; d < 3; ++d) {  // streaming stores: nothing reads them before the next launch
+            typedef float nf4 __attribute__((ext_vector_type(4)));
+            const nf4 v = {P1[d].x, P1[d].y, P1[d].z, P1[d].w};
+            __builtin_nontemporal_store(v, reinterpret_cast<nf4 *>(out[d] + off));
+          }
         }
       }
+      __syncthreads();
+      copy3(P0, P1);
+      copy3(P1, P2);
+      copy3(P2, P3);
+      copy3(P3, P4);
     }
-    __syncthreads();
-    copy3(P0, P1);
-    copy3(P1, P2);
-    copy3(P2, P3);
-    copy3(P3, P4);
-  }
+  };
+  if (SR != 64)
+    walk(std::integral_constant<int, 0>{});
+  else if (owned_row)
+    walk(std::integral_constant<int, 2>{});
+  else
+    walk(std::integral_constant<int, 1>{});
   return acc;
 }
 
