@@ -164,9 +164,9 @@ def learned_leg(budget_s=30.0, n_train=1_000_000, L5=128, temps=(1.10, 1.12, 1.1
                 with mclics.Lattice(mclics.Config(L5, L5, L5, T=T, seed=90 + ti), starts[T]) as lat:
                     if method == "learned":
                         lat.set_learned(net, reflect=reflect)
-                        lat.sweeps(200)
-                        ms, _ = lat.time_sweeps(300, graph=False)
-                        sps = 300 / (ms / 1e3)
+                        lat.sweeps(200, graph=True)
+                        ms, _ = lat.time_sweeps(1000, graph=True)
+                        sps = 1000 / (ms / 1e3)
                     else:
                         lat.autotune()  # the plain chain at its fastest kernel shape for this size
                         lat.sweeps(200, graph=True)

@@ -140,7 +140,10 @@ int mc_sweeps(mc_handle *h, int n);
 /* Same, through captured hipGraphs of at most 64 sweeps each: the graph of
  * 64 sweeps is replayed n / 64 times, then the graph of the n % 64 that
  * remain.  Graphs are captured on first use and cached per handle (at most
- * 128 of them); the chain equals mc_sweeps's bit for bit. */
+ * 128 of them); the chain equals mc_sweeps's bit for bit.  With a learned
+ * proposal attached, sweeps are captured when every sweep is learned
+ * (every = 1) and run eagerly otherwise; attaching a network or switching
+ * its reflection move drops the cached graphs. */
 int mc_sweeps_graph(mc_handle *h, int n);
 /* Blocks until queued work is done. */
 int mc_sync(mc_handle *h);

@@ -188,6 +188,11 @@ bool learned_sweep(const mc_handle *h) {
   return h->learned && h->learned_every > 0 && (h->host_sweeps % h->learned_every) == 0;
 }
 
+// Sweeps can be captured in graphs: plain sweeps, or learned sweeps only
+// (every = 1).  With plain and learned sweeps interleaved, which sweep of a
+// graph is learned would depend on where the graph starts.
+bool graphable(const mc_handle *h) { return !h->learned || h->learned_every == 1; }
+
 int passes_per_sweep(const mc_handle *h) {
   if (h->order == MC_ORDER_REF8) return 8;
   return use_slab(h) && !learned_sweep(h) ? 1 : 2;
@@ -291,11 +296,11 @@ void launch_measure(mc_handle *h, double *series) {
     mc_launch_measure(L, h->measure_grid, h->partials, h->result, series);
 }
 
-// Graph of n plain sweeps starting from the current plane set.  Capture
+// Graph of n sweeps starting from the current plane set.  Capture
 // records the launches without running them, so the host-side plane swaps
 // and sweep count made while capturing are undone afterwards.
 int sweep_graph(mc_handle *h, int n, hipGraphExec_t *out) {
-  const int key = 2 * n + h->flip;
+  const int key = 4 * n + 2 * (h->learned ? 1 : 0) + h->flip;
   auto it = h->graphs.find(key);
   if (it != h->graphs.end()) {
     *out = it->second;
@@ -328,7 +333,7 @@ int sweep_graph(mc_handle *h, int n, hipGraphExec_t *out) {
 // way the captured launches advance the device.
 int launch_sweep_graph(mc_handle *h, int n, hipGraphExec_t ge) {
   MC_HIP(hipGraphLaunch(ge, h->stream));
-  if (use_slab(h) && (n & 1)) swap_planes(h);
+  if (use_slab(h) && !h->learned && (n & 1)) swap_planes(h);  // learned sweeps keep the plane set
   h->host_sweeps += n;
   return MC_OK;
 }
@@ -913,7 +918,7 @@ int mc_sweeps_graph(mc_handle *h, int n) {
   if (rc) return rc;
   if (n < 0) return fail(MC_EINVAL, "mc_sweeps_graph: negative sweep count");
   if (n == 0) return MC_OK;
-  if (h->learned) return mc_sweeps(h, n);  // the learned/plain pattern depends on the sweep count
+  if (!graphable(h)) return mc_sweeps(h, n);
   GraphPlan g;
   rc = plan_sweep_graphs(h, n, &g);
   if (rc) return rc;
@@ -931,7 +936,7 @@ int mc_time_sweeps(mc_handle *h, int n, int use_graph, float *ms, int *launches)
   int rc = check_handle(h);
   if (rc) return rc;
   if (n < 1 || !ms) return fail(MC_EINVAL, "mc_time_sweeps: need n >= 1 and an output");
-  if (use_graph && h->learned) use_graph = 0;
+  if (use_graph && !graphable(h)) use_graph = 0;
   GraphPlan g;
   if (use_graph) {
     // Instantiate before timing so capture cost stays outside the region.
@@ -1171,6 +1176,7 @@ int mc_set_learned_proposal(mc_handle *h, mc_mlp *net, int every) {
       return fail(MC_EINVAL, "mc_set_learned_proposal: network and lattice live on different devices");
   }
   MC_HIP(hipStreamSynchronize(h->stream));
+  drop_graphs(h);  // captured sweeps bake in the proposal
   h->learned = every > 0 ? net : nullptr;
   h->learned_every = net ? every : 0;
   h->host_sweeps = 0;
@@ -1182,6 +1188,7 @@ int mc_set_learned_reflection(mc_handle *h, int on) {
   if (rc) return rc;
   if (on != 0 && on != 1) return fail(MC_EINVAL, "mc_set_learned_reflection: on must be 0 or 1");
   MC_HIP(hipStreamSynchronize(h->stream));
+  if (on != h->learned_reflect) drop_graphs(h);  // captured learned sweeps bake in the switch
   h->learned_reflect = on;
   return MC_OK;
 }

@@ -285,3 +285,42 @@ def test_learned_and_plain_agree_within_3_sigma():
             (Ea, ea, Sa, sa), (Eb, eb, Sb, sb) = res["plain"], res["learned"]
             assert abs(Ea - Eb) <= 3 * math.hypot(ea, eb), (T, res)
             assert abs(Sa - Sb) <= 3 * math.hypot(sa, sb), (T, res)
+
+
+@pytest.mark.parametrize("kernel", ["auto", "slab"])
+def test_learned_sweeps_in_graphs_equal_eager_sweeps(kernel):
+    # Captured learned sweeps (a 64-sweep chunk replayed, then a remainder)
+    # give the eager chain bit for bit, before and after the reflection move
+    # is switched off (which drops the captured graphs).  The slab kernel's
+    # plane swaps must not be replayed for learned sweeps, which keep the
+    # plane set; plain graph sweeps after the network is detached must also
+    # match.
+    L = 16
+    with mclics.Network(seed=12) as net:
+        src = equilibrated(L, 1.1, 100, 51)
+        src.sweeps(10)
+        net.add_samples(src, 8192, seed=1200)
+        n0 = src.directors()
+        src.close()
+        net.train(epochs=2, batch=1024, lr=2e-3)
+        lats = []
+        for graph in (False, True):
+            lt = mclics.Lattice(mclics.Config(L, L, L, T=1.1, seed=52), n0)
+            if kernel == "slab":
+                lt.set_kernel(mclics.KERNEL_SLAB)
+            lt.sweeps(3, graph=graph)  # plain sweeps first (the slab kernel flips the plane set)
+            lt.set_learned(net)
+            lt.sweeps(64 + 5, graph=graph)
+            lt.set_learned(net, reflect=False)
+            lt.sweeps(7, graph=graph)
+            lt.set_learned(net)
+            lt.sweeps(3, graph=graph)
+            lt.set_learned(None)
+            lt.sweeps(5, graph=graph)
+            lt.sync()
+            lats.append(lt)
+        a, b = lats
+        assert np.array_equal(a.directors(), b.directors())
+        assert a.acceptance() == b.acceptance()
+        for lt in lats:
+            lt.close()
