@@ -288,7 +288,7 @@ __device__ __forceinline__ float lcosh2(float z) {
 
 __device__ __forceinline__ void head(const float *o, const float base[3], float mu[3], float &logk) {
   float v0 = base[0] + o[0], v1 = base[1] + o[1], v2 = base[2] + o[2];
-  float r2 = v0 * v0 + v1 * v1 + v2 * v2;
+  float r2 = fmaf(v2, v2, fmaf(v1, v1, v0 * v0));
   const float inv = r2 > 0.0f ? rsqrtf(r2) : 1.0f;
   mu[0] = v0 * inv;
   mu[1] = v1 * inv;
@@ -704,21 +704,22 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
   const float v = 1.0f - u_w;  // (0, 1]
   // (hardware reciprocal and square root, 1 ulp: the draw is a sample, and
   // the Metropolis-Hastings ratio uses the density, not these operations)
-  float w = 1.0f + __logf(v + (1.0f - v) * __expf(-2.0f * kappa)) * __builtin_amdgcn_rcpf(kappa);
+  float w = fmaf(__logf(fmaf(1.0f - v, __expf(-2.0f * kappa), v)), __builtin_amdgcn_rcpf(kappa), 1.0f);
   w = fminf(fmaxf(w, -1.0f), 1.0f);
-  const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, fmaf(-w, w, 1.0f)));
   // orthonormal e, f perpendicular to the unit mu, branch free (Duff et al.,
   // "Building an orthonormal basis, revisited", JCGT 6(1), 2017)
   const float sgn = mu[2] >= 0.0f ? 1.0f : -1.0f;
   const float ia = -__builtin_amdgcn_rcpf(sgn + mu[2]);
   const float bb = mu[0] * mu[1] * ia;
-  const float e0 = 1.0f + sgn * mu[0] * mu[0] * ia, e1 = sgn * bb, e2 = -sgn * mu[0];
-  const float f0 = bb, f1 = sgn + mu[1] * mu[1] * ia, f2 = -mu[1];
+  const float si = sgn * ia;
+  const float e0 = fmaf(si * mu[0], mu[0], 1.0f), e1 = sgn * bb, e2 = -sgn * mu[0];
+  const float f0 = bb, f1 = fmaf(mu[1] * mu[1], ia, sgn), f2 = -mu[1];
   const float ph = u_phi;  // revolutions
   const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
-  float q0 = w * mu[0] + st * (cs * e0 + sn * f0);
-  float q1 = w * mu[1] + st * (cs * e1 + sn * f1);
-  float q2 = w * mu[2] + st * (cs * e2 + sn * f2);
+  float q0 = fmaf(st, fmaf(cs, e0, sn * f0), w * mu[0]);
+  float q1 = fmaf(st, fmaf(cs, e1, sn * f1), w * mu[1]);
+  float q2 = fmaf(st, fmaf(cs, e2, sn * f2), w * mu[2]);
   // |q|^2 = 1 + d with d a few ulps: q (1.5 - 0.5 |q|^2) is unit to second order
   const float iq = fmaf(-0.5f, fmaf(q2, q2, fmaf(q1, q1, q0 * q0)), 1.5f);
   const float sg = neg ? -iq : iq;
@@ -784,14 +785,15 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     sample_axial_vmf(mu, kappa, mcd_u01(r.x), mcd_u01(r.y), (r.z & 1u) != 0u, q);
     const float o0 = valid ? nx[s] : 1.0f, o1 = valid ? ny[s] : 0.0f, o2 = valid ? nz[s] : 0.0f;
     float dE;
+    float dot_o[6], dot_q[6];  // director . neighbour for the old and the proposed director (LL)
     if (POT == MCD_LL) {
       float a2 = 0.0f;
 #pragma unroll
       for (int bb = 0; bb < 6; ++bb) {
+        dot_o[bb] = fmaf(o2, m[bb][2], fmaf(o1, m[bb][1], o0 * m[bb][0]));
+        dot_q[bb] = fmaf(q[2], m[bb][2], fmaf(q[1], m[bb][1], q[0] * m[bb][0]));
         if (!on[bb]) continue;
-        const float a = fmaf(o2, m[bb][2], fmaf(o1, m[bb][1], o0 * m[bb][0]));
-        const float c = fmaf(q[2], m[bb][2], fmaf(q[1], m[bb][1], q[0] * m[bb][0]));
-        a2 = fmaf(c - a, c + a, a2);
+        a2 = fmaf(dot_q[bb] - dot_o[bb], dot_q[bb] + dot_o[bb], a2);
       }
       dE = cAf * a2;
     } else {
@@ -805,10 +807,10 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
       dE = (float)sn - (float)so;
     }
     if (P.elecA != 0.0f) dE += mcd_field(P, E, q[0], q[1], q[2]) - mcd_field(P, E, o0, o1, o2);
-    const float co = mu[0] * o0 + mu[1] * o1 + mu[2] * o2;
-    const float cn = mu[0] * q[0] + mu[1] * q[1] + mu[2] * q[2];
+    const float co = fmaf(mu[2], o2, fmaf(mu[1], o1, mu[0] * o0));
+    const float cn = fmaf(mu[2], q[2], fmaf(mu[1], q[1], mu[0] * q[0]));
     const float lr = lcosh2(kappa * co) - lcosh2(kappa * cn);
-    const float xlog = -dE * invT + lr;
+    const float xlog = fmaf(-dE, invT, lr);
     const float u = mcd_u01(r.w);
     const bool a = valid && u < __expf(fminf(xlog, 80.0f));
     // The reflection move from the director c the first move left.  Its
@@ -819,10 +821,10 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
     float r0 = c0, r1 = c1, r2 = c2, dE2 = 0.0f, u2 = 0.0f;
     bool a2 = false;
     if (reflect) {
-      const float dm = mu[0] * c0 + mu[1] * c1 + mu[2] * c2;
-      r0 = 2.0f * dm * mu[0] - c0;
-      r1 = 2.0f * dm * mu[1] - c1;
-      r2 = 2.0f * dm * mu[2] - c2;
+      const float dm2 = 2.0f * fmaf(mu[2], c2, fmaf(mu[1], c1, mu[0] * c0));
+      r0 = fmaf(dm2, mu[0], -c0);
+      r1 = fmaf(dm2, mu[1], -c1);
+      r2 = fmaf(dm2, mu[2], -c2);
       const float rn = fmaf(-0.5f, fmaf(r2, r2, fmaf(r1, r1, r0 * r0)), 1.5f);
       r0 *= rn;
       r1 *= rn;
@@ -832,7 +834,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
 #pragma unroll
         for (int bb = 0; bb < 6; ++bb) {
           if (!on[bb]) continue;
-          const float ac = fmaf(c2, m[bb][2], fmaf(c1, m[bb][1], c0 * m[bb][0]));
+          const float ac = a ? dot_q[bb] : dot_o[bb];  // c . m, computed for the first move
           const float ar = fmaf(r2, m[bb][2], fmaf(r1, m[bb][1], r0 * m[bb][0]));
           b2 = fmaf(ar - ac, ar + ac, b2);
         }
