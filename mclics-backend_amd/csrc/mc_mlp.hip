@@ -214,7 +214,12 @@ __device__ __forceinline__ void load_weights(const MLPDev &N, bf16x8 w1[4], bf16
   w3[1] = bop(N.W3b, 64, 0, 32);
 }
 
-// Features of one site, FP32 operation for operation as orc_mlp_features.
+// Features of one site, as orc_mlp_features.  The dot products that pick
+// each neighbour's sign are fused multiply-adds here: only their signs are
+// used, and those can differ from the unfused products only when d is a few
+// ulps from zero, where either sign is a valid choice (a director and its
+// negative are the same state).  Training samples and learned sweeps both
+// take their features from this function.
 __device__ __forceinline__ void features(const float m[6][3], float f0, float f1, float f2, float T, float x[22],
                                          float base[3]) {
   float c0 = m[0][0], c1 = m[0][1], c2 = m[0][2];
@@ -232,9 +237,7 @@ __device__ __forceinline__ void features(const float m[6][3], float f0, float f1
   base[2] = c2;
 #pragma unroll
   for (int b = 1; b < 6; ++b) {
-    float d = c0 * m[b][0];
-    d = d + c1 * m[b][1];
-    d = d + c2 * m[b][2];
+    const float d = fmaf(c2, m[b][2], fmaf(c1, m[b][1], c0 * m[b][0]));
     const float s = d < 0.0f ? -1.0f : 1.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -773,7 +776,9 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
       gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
       float x[22];
       features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
-      write_xrow24(Xs, l, x, valid);
+      // (a lane past the end holds site 0's features: the forward pass keeps
+      // rows apart, and that lane's move is never taken)
+      write_xrow24(Xs, l, x, true);
     }
     wave_sync();
     forward64l(Wl, Bl, Xs, Os);
