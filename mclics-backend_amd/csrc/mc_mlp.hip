@@ -280,13 +280,18 @@ __device__ __forceinline__ void write_xrow24(__bf16 *Xs, int row, const float x[
   }
 }
 
+// Natural log of a normal positive float with the hardware log2
+// (v_log_f32, within one ulp) and one multiply: two instructions where the
+// library logf (with its denormal scaling and split ln 2) takes eleven.
+__device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+
 // log cosh(z) + log 2, FP32.
 // log(1 + y) for y = e^{-2|z|} in (0, 1] with the hardware log: 1 + y is
 // rounded once (absolute error <= 2^-24 in the result), where the library
 // log1pf costs ~110 instructions per call.
 __device__ __forceinline__ float lcosh2(float z) {
   const float a = fabsf(z);
-  return a + __logf(1.0f + __expf(-2.0f * a));
+  return a + hw_log(1.0f + __builtin_amdgcn_exp2f(a * -2.88539008f));  // e^{-2a} = 2^{-2a log2 e}
 }
 
 __device__ __forceinline__ void head(const float *o, const float base[3], float mu[3], float &logk) {
@@ -707,7 +712,9 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
   const float v = 1.0f - u_w;  // (0, 1]
   // (hardware reciprocal and square root, 1 ulp: the draw is a sample, and
   // the Metropolis-Hastings ratio uses the density, not these operations)
-  float w = fmaf(__logf(fmaf(1.0f - v, __expf(-2.0f * kappa), v)), __builtin_amdgcn_rcpf(kappa), 1.0f);
+  // (the log's argument is at least v >= 2^-24, a normal float)
+  float w = fmaf(hw_log(fmaf(1.0f - v, __builtin_amdgcn_exp2f(kappa * -2.88539008f), v)),
+                 __builtin_amdgcn_rcpf(kappa), 1.0f);
   w = fminf(fmaxf(w, -1.0f), 1.0f);
   const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, fmaf(-w, w, 1.0f)));
   // orthonormal e, f perpendicular to the unit mu, branch free (Duff et al.,
