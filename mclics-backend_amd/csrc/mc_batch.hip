@@ -195,6 +195,15 @@ bool slab_plan(const mc_batch *b, int budget, int cz_target, bool aligned, std::
       for (int y0 = 0; y0 < P.ny; y0 += ty) wg.push_back(int4{r, y0, z0, 0});
   }
   *threads = (*threads + 63) / 64 * 64;
+  // Workgroup b runs on XCD b % 8: each XCD takes a contiguous run of the
+  // table, so tiles of one replica that run side by side share their halo
+  // rows in one L2.
+  const int n = (int)wg.size();
+  int start[9] = {0};
+  for (int x = 0; x < 8; ++x) start[x + 1] = start[x] + (n - x + 7) / 8;  // workgroups b < n with b % 8 == x
+  std::vector<int4> perm((size_t)n);
+  for (int b = 0; b < n; ++b) perm[(size_t)b] = wg[(size_t)(start[b % 8] + b / 8)];
+  wg.swap(perm);
   return true;
 }
 
