@@ -758,7 +758,7 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
   __syncthreads();
   const __bf16 *Wl = reinterpret_cast<const __bf16 *>(Wst);
   const float *Bl = reinterpret_cast<const float *>(Wst + 14 * 64);
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   __bf16 *Xs = Xall[wave];
   float *Os = Oall[wave];
 
@@ -769,18 +769,35 @@ __global__ __launch_bounds__(64 * kLearnWaves) __attribute__((amdgpu_waves_per_e
   const float cAf = (float)cA;
   const int l = threadIdx.x & 63;
   const int total = mcd_set_slots(P, 8 + colour);
+  // Periodic lattice with Nx/2 a multiple of 64: a block is 64 consecutive
+  // colour sites of one row, so its row (j, k) is one scalar for the wave.
+  const bool rowblk = PER && ((P.nx >> 1) & 63) == 0;
   uint32_t acc = 0;
   for (int blk = blockIdx.x * kLearnWaves + wave; blk * 64 < total; blk += gridDim.x * kLearnWaves) {
     const int t = blk * 64 + l;
-    int i = 0, j = 0, k = 0;
-    bool valid = t < total && mcd_set_site(P, 8 + colour, t, i, j, k);
-    if (!valid) i = j = k = 0;  // a slot past an odd free Nx: gather in-bounds, result unused
-    const int s = valid ? i + P.nx * (j + P.ny * k) : 0;
+    bool valid;
+    int s;
     float m[6][3];
     bool on[6];
     float b[3];
-    {
+    if (rowblk) {
+      // (its own gather, so the row's offsets stay scalar)
+      const int t0 = blk * 64;
+      const int row = (int)mcd_div((uint32_t)t0, P.div_hx);
+      const int k = (int)mcd_div((uint32_t)row, P.div_ny);
+      const int j = row - k * P.ny;
+      const int i = 2 * (t0 - row * (int)P.div_hx.d + l) + ((j + k + colour) & 1);
+      valid = true;  // every slot of a block is a site: total is a multiple of 64
+      s = i + P.nx * (j + P.ny * k);
       gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
+    } else {
+      int i = 0, j = 0, k = 0;
+      valid = t < total && mcd_set_site(P, 8 + colour, t, i, j, k);
+      if (!valid) i = j = k = 0;  // a slot past an odd free Nx: gather in-bounds, result unused
+      s = valid ? i + P.nx * (j + P.ny * k) : 0;
+      gather6<PER>(P, nx, ny, nz, i, j, k, m, on);
+    }
+    {
       float x[22];
       features(m, E * P.eX, E * P.eY, E * P.eZ, T, x, b);
       // (a lane past the end holds site 0's features: the forward pass keeps

@@ -129,7 +129,7 @@ def mh_window(dE_T, kap, lk, co, cn, x):
 
 
 @pytest.mark.parametrize("dims,bound,reflect", [((12, 12, 12), (1, 1, 1), True), ((7, 6, 6), (0, 1, 1), True),
-                                                ((12, 12, 12), (1, 1, 1), False)])
+                                                ((12, 12, 12), (1, 1, 1), False), ((128, 6, 4), (1, 1, 1), True)])
 def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect):
     # Replays one recorded learned sweep: the oracle recomputes the energy
     # change from the reference site energies (the recorded FP32 dE must be
@@ -142,7 +142,8 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect):
     # its dE the reference's, and its decision u2 < exp(-dE2/T) outside the
     # FP32 window of -dE2 RN(1/T) and the hardware exp.
     # The 7 x 6 x 6 lattice with a free x axis has colour-set slots past the
-    # last column.
+    # last column.  On the 128 x 6 x 4 lattice a 64-site block is one row's
+    # colour sites, which the pass decodes and gathers with scalar row offsets.
     nx, ny, nz = dims
     T = 1.1
     rng = np.random.default_rng(4)
