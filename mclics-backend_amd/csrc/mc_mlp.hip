@@ -297,11 +297,13 @@ __device__ __forceinline__ float lcosh2(float z) {
 __device__ __forceinline__ void head(const float *o, const float base[3], float mu[3], float &logk) {
   float v0 = base[0] + o[0], v1 = base[1] + o[1], v2 = base[2] + o[2];
   float r2 = fmaf(v2, v2, fmaf(v1, v1, v0 * v0));
-  const float inv = r2 > 0.0f ? rsqrtf(r2) : 1.0f;
+  // hardware reciprocal square root (1 ulp) on a normal r2; mu is a proposal
+  // parameter, used alike in both directions of the move
+  const float inv = r2 > 1e-30f ? __builtin_amdgcn_rsqf(r2) : 1.0f;
   mu[0] = v0 * inv;
   mu[1] = v1 * inv;
   mu[2] = v2 * inv;
-  logk = fminf(fmaxf(o[3], -6.0f), 8.0f);
+  logk = __builtin_amdgcn_fmed3f(o[3], -6.0f, 8.0f);  // clamp to [-6, 8]
 }
 
 // ---------------------------------------------------------------------------
@@ -713,7 +715,7 @@ __device__ __forceinline__ void sample_axial_vmf(const float mu[3], float kappa,
   // (hardware reciprocal and square root, 1 ulp: the draw is a sample, and
   // the Metropolis-Hastings ratio uses the density, not these operations)
   // (the log's argument is at least v >= 2^-24, a normal float)
-  float w = fmaf(hw_log(fmaf(1.0f - v, __builtin_amdgcn_exp2f(kappa * -2.88539008f), v)),
+  float w = fmaf(hw_log(fmaf(u_w, __builtin_amdgcn_exp2f(kappa * -2.88539008f), v)),  // 1 - v = u_w exactly
                  __builtin_amdgcn_rcpf(kappa), 1.0f);
   w = fminf(fmaxf(w, -1.0f), 1.0f);
   const float st = __builtin_amdgcn_sqrtf(fmaxf(0.0f, fmaf(-w, w, 1.0f)));
