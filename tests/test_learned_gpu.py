@@ -128,9 +128,19 @@ def mh_window(dE_T, kap, lk, co, cn, x):
     return 2.0 * (2 * abs(dE_T) * U32 + zerr + lc + U32 * (zo + zn) + (2 * abs(x) + 3) * U32)
 
 
-@pytest.mark.parametrize("dims,bound,reflect", [((12, 12, 12), (1, 1, 1), True), ((7, 6, 6), (0, 1, 1), True),
-                                                ((12, 12, 12), (1, 1, 1), False), ((128, 6, 4), (1, 1, 1), True)])
-def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect):
+GHRL_K = (1.0, 0.7, 1.3, 16.0)
+PHYS = {"ll": {},
+        "ghrl_field": {"potential": O.GHRL, "k": GHRL_K, "elec": (0.3, -0.2, 1.1), "elecA": 0.8, "elecE": 0.7},
+        "pear": {"potential": O.PEAR, "A": 1.0, "B1": 0.3, "Cc": 0.3}}
+
+
+@pytest.mark.parametrize("dims,bound,reflect,phys", [((12, 12, 12), (1, 1, 1), True, "ll"),
+                                                     ((7, 6, 6), (0, 1, 1), True, "ll"),
+                                                     ((12, 12, 12), (1, 1, 1), False, "ll"),
+                                                     ((128, 6, 4), (1, 1, 1), True, "ll"),
+                                                     ((8, 6, 4), (1, 1, 1), True, "ghrl_field"),
+                                                     ((7, 6, 6), (0, 1, 1), True, "pear")])
+def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect, phys):
     # Replays one recorded learned sweep: the oracle recomputes the energy
     # change from the reference site energies (the recorded FP32 dE must be
     # within the FMA form's proven distance of it) and, in FP64 from the
@@ -144,14 +154,18 @@ def test_learned_sweep_decisions_are_exact_mh(dims, bound, reflect):
     # The 7 x 6 x 6 lattice with a free x axis has colour-set slots past the
     # last column.  On the 128 x 6 x 4 lattice a 64-site block is one row's
     # colour sites, which the pass decodes and gathers with scalar row offsets.
+    # GHRL (with a field) and pear take the FP64 bond sums of the site energy
+    # for both moves.
     nx, ny, nz = dims
     T = 1.1
     rng = np.random.default_rng(4)
     P = rand_params(rng, 0.1)
-    po = O.params(nx, ny, nz, bound=bound)
+    po = O.params(nx, ny, nz, bound=bound, **PHYS[phys])
     pt = np.ones(nx * ny * nz, np.int32)
-    with mclics.Network(P) as net, mclics.Lattice(mclics.Config(nx, ny, nz, bound=bound, T=T, seed=5),
-                                                  mclics.random_ic(nx, ny, nz, seed=5)) as lat:
+    cfg = mclics.Config(nx, ny, nz, bound=bound, potential=po.potential, A=po.A, B1=po.B1, B2=po.B2, C=po.C,
+                        ghrl=O.ghrl_tuple(po), elec=(po.elecX, po.elecY, po.elecZ), elecA=po.elecA,
+                        elecE=po.elecE, T=T, seed=5)
+    with mclics.Network(P) as net, mclics.Lattice(cfg, mclics.random_ic(nx, ny, nz, seed=5)) as lat:
         lat.sweeps(50)
         start = lat.directors()
         lat.set_learned(net, reflect=reflect)
