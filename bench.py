@@ -34,7 +34,7 @@ def log(*a):
 
 def host_cores():
     """(threads, description) for the CPU legs: OMP_NUM_THREADS when it is
-    set (the GPU box sets it to the process's CPU share), else every core in
+    set (a scheduler may set it to the process's CPU share), else every core in
     this process's affinity mask; with the host CPU model."""
     avail = len(os.sched_getaffinity(0))
     env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)

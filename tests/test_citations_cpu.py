@@ -14,14 +14,21 @@ PAT = re.compile(r"\b((?:src|include)/[A-Za-z0-9_./]+?\.(?:cpp|cuh|cu|h|bak))(?!
 OWN = {"include/mclics_hip.h"}  # this repository's own header
 
 
+# what this repository ships: the package, the oracle, the header, the tests,
+# the profiles, the bench and the three documents of its own
+ROOTS = ("mclics-backend_amd", "oracle", "include", "tests", "profiles")
+DOCS = ("DESIGN.md", "INTEGRATION.md", "README.md", "bench.py", "__graft_entry__.py")
+
+
 def cited_files():
-    skip = ("SURVEY.md", "BASELINE.md", "VERDICT.md", "ADVICE.md")
-    for f in glob.glob(os.path.join(REPO, "**", "*"), recursive=True):
-        rel = os.path.relpath(f, REPO)
-        if rel.startswith(("_work", "gpurun_out", ".git")) or os.path.basename(f) in skip:
-            continue
-        if os.path.isfile(f) and f.endswith((".c", ".h", ".cpp", ".hip", ".py", ".md")):
-            yield rel, f
+    for root in ROOTS:
+        for f in glob.glob(os.path.join(REPO, root, "**", "*"), recursive=True):
+            if os.path.isfile(f) and f.endswith((".c", ".h", ".cpp", ".hip", ".py", ".md")):
+                yield os.path.relpath(f, REPO), f
+    for name in DOCS:
+        f = os.path.join(REPO, name)
+        if os.path.isfile(f):
+            yield name, f
 
 
 def test_citations_point_inside_the_reference_files():
