@@ -658,8 +658,12 @@ __global__ __launch_bounds__(kMaxThreads) void k_measure_slab(MCDevParams P, con
                                                               const float *__restrict__ inz, const MCDevState *st,
                                                               int TY, int CZ, double *__restrict__ partials) {
   extern __shared__ float lds[];
-  const int nyt = P.ny / TY;
-  const int y0 = ((int)blockIdx.x % nyt) * TY, z0 = ((int)blockIdx.x / nyt) * CZ;
+  const int nyt = P.ny / TY, ntiles = nyt * (P.nz / CZ);
+  // Workgroup b runs on XCD b % 8: each XCD takes a contiguous run of tiles
+  // (neighbouring rows share their halo rows in one L2)
+  const int b = (int)blockIdx.x;
+  const int tile = (ntiles & 7) == 0 ? (b & 7) * (ntiles >> 3) + (b >> 3) : b;
+  const int y0 = (tile % nyt) * TY, z0 = (tile / nyt) * CZ;
   const float *const in[3] = {inx, iny, inz};
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   measure_slab_tile<POT, GEO>(P, in, st->elecE, TY, CZ, y0, z0, lds, a);
